@@ -1,0 +1,195 @@
+/*
+ * fbfuse_hip.h -- C ABI of the MI355X-native FBFUSE beamformer.
+ *
+ * Plain C: pointers, sizes and status codes; no HIP, thrust or torch types in
+ * any signature.  Streams are passed as opaque `fbf_stream_t` (a hipStream_t;
+ * NULL is the default stream).  Every function returns FBF_OK (0) or a
+ * non-zero status; fbf_last_error() gives the message of the last failure on
+ * the calling thread.  No exception crosses the ABI.
+ *
+ * Compute entry points are ASYNCHRONOUS on the given stream (they only
+ * enqueue work); the C++ operator classes above this ABI synchronise the
+ * stream on return, which restores the reference's synchronous contract.
+ *
+ * Data layouts (char2 = one complex int8 sample, re then im; npol = 2):
+ *   TAFTP voltages  [heap_group][antenna][channel][sample_in_heap][pol] char2
+ *   FTPA voltages   [channel][sample][pol][antenna] char2
+ *   FBA weights     [channel][beam][antenna] char2
+ *   TBTF CB output  [sample_out / ohs][beam][sample_out % ohs][channel_out] int8
+ *   TF IB output    [sample_out][channel_out] int8
+ *   delays          [beam][antenna] float2 (x = rate in s/s, y = offset in s)
+ *
+ * Reference interfaces replaced (paths relative to the psrdada_cpp tree):
+ *   fbf_split_transpose   <- SplitTranspose::transpose, SplitTranspose.cuh:73-74,
+ *                            kernel SplitTranspose.cu:11-70
+ *   fbf_cb_*              <- CoherentBeamformer::beamform, CoherentBeamformer.cuh:128-131,
+ *                            kernel CoherentBeamformer.cu:48-170
+ *   fbf_ib_*              <- IncoherentBeamformer::beamform, IncoherentBeamformer.cuh:52-54,
+ *                            kernel IncoherentBeamformer.cu:10-69
+ *   fbf_weights_*         <- WeightsManager::weights, WeightsManager.cuh:69,
+ *                            kernel WeightsManager.cu:14-79
+ *   fbf_power_levels      <- PipelineConfig::update_power_offsets_and_scalings,
+ *                            PipelineConfig.cpp:109-127
+ *   fbf_channel_frequencies <- PipelineConfig::calculate_channel_frequencies,
+ *                            PipelineConfig.cpp:180-195
+ *   fbf_malloc/fbf_memcpy_* <- thrust::device_vector storage used by
+ *                            DoubleDeviceBuffer, double_device_buffer.cuh:9-10
+ */
+#ifndef FBFUSE_HIP_H
+#define FBFUSE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* fbf_stream_t;
+typedef void* fbf_event_t;
+
+enum fbf_status {
+    FBF_OK = 0,
+    FBF_ERR_ARG = 1,          /* bad argument (null pointer, bad enum, bad shape) */
+    FBF_ERR_SIZE = 2,         /* buffer size does not match the geometry */
+    FBF_ERR_HIP = 3,          /* HIP runtime error */
+    FBF_ERR_UNSUPPORTED = 4,  /* geometry outside what the kernels support */
+    FBF_ERR_STATE = 5,        /* call not valid in the object's current state */
+    FBF_ERR_CAPACITY = 6      /* admission refused: budget or queue full */
+};
+
+/* Quantiser of the detected power (CoherentBeamformer.cu:169):
+ * ROUND: clamp(round_half_away((P - offset) / scale), -128, 127)
+ * TRUNC: clamp(trunc((P - offset) / scale), -128, 127) -- the reference cast.
+ * (P - offset) / scale is evaluated in IEEE double from the exact power. */
+enum fbf_quant_mode { FBF_QUANT_ROUND = 0, FBF_QUANT_TRUNC = 1 };
+
+const char* fbf_last_error(void);
+int fbf_version(void);
+
+/* ---- device plumbing (hipMalloc & co; for callers without their own HIP) */
+int fbf_device_count(int* count);
+int fbf_set_device(int device);
+int fbf_device_name(char* buf, size_t buflen);
+int fbf_malloc(void** ptr, size_t bytes);
+int fbf_free(void* ptr);
+int fbf_host_malloc(void** ptr, size_t bytes);      /* pinned host memory */
+int fbf_host_free(void* ptr);
+int fbf_memcpy_h2d(void* dst, const void* src, size_t bytes, fbf_stream_t stream);
+int fbf_memcpy_d2h(void* dst, const void* src, size_t bytes, fbf_stream_t stream);
+int fbf_memcpy_d2d(void* dst, const void* src, size_t bytes, fbf_stream_t stream);
+int fbf_memset(void* dst, int value, size_t bytes, fbf_stream_t stream);
+int fbf_stream_create(fbf_stream_t* stream);
+int fbf_stream_destroy(fbf_stream_t stream);
+int fbf_stream_synchronize(fbf_stream_t stream);
+int fbf_device_synchronize(void);
+int fbf_event_create(fbf_event_t* event);
+int fbf_event_destroy(fbf_event_t event);
+int fbf_event_record(fbf_event_t event, fbf_stream_t stream);
+int fbf_event_synchronize(fbf_event_t event);
+int fbf_event_elapsed_ms(fbf_event_t start, fbf_event_t stop, float* ms);
+
+/* ---- deterministic synthetic data (host, splitmix64 + Box-Muller) ------ */
+/* re/im = round(N(0, sigma)); clip != 0: clamp to [-127, 127], else wrap to
+ * int8 like static_cast<int8_t>(lround(x)).  Writes n bytes. */
+int fbf_synth_gaussian_int8(uint64_t seed, double sigma, int clip, int8_t* out, size_t n);
+/* n floats uniform in [lo, hi). */
+int fbf_synth_uniform_f32(uint64_t seed, float lo, float hi, float* out, size_t n);
+
+/* ---- configuration helpers -------------------------------------------- */
+/* PipelineConfig.cpp:109-127 (same float arithmetic). */
+int fbf_power_levels(float input_level, float output_level,
+                     int cb_nantennas, int cb_tscrunch, int cb_fscrunch,
+                     int ib_nantennas, int ib_tscrunch, int ib_fscrunch, int npol,
+                     float* cb_offset, float* cb_scale,
+                     float* ib_offset, float* ib_scale);
+/* PipelineConfig.cpp:180-195 (float) and the same centres in double. */
+int fbf_channel_frequencies(float cfreq, float bw, int nchans, float* freqs);
+int fbf_channel_frequencies_f64(double cfreq, double bw, int nchans, double* freqs);
+
+/* ---- split transpose (SplitTranspose.cu:11-126) ------------------------ */
+/* TAFTP -> FTPA for the contiguous antenna subset
+ * [start_antenna, start_antenna + used_nantennas). ftpa_bytes must equal
+ * 2*2*used_nantennas*nchans*nheap_groups*nsamples_per_heap. */
+int fbf_split_transpose(const void* taftp, size_t taftp_bytes,
+                        void* ftpa, size_t ftpa_bytes,
+                        int total_nantennas, int used_nantennas, int start_antenna,
+                        int nchans, int nsamples_per_heap, fbf_stream_t stream);
+
+/* ---- coherent beamformer (CoherentBeamformer.cu:48-225) ---------------- */
+typedef struct fbf_cb_desc {
+    int nantennas;              /* antennas in the FTPA input (A) */
+    int nbeams;                 /* beams (B), multiple of 8 */
+    int nchans;                 /* channels (F) */
+    int tscrunch;               /* time integration */
+    int fscrunch;               /* channel integration */
+    int nsamples_per_heap_out;  /* ohs: output samples per TBTF heap; 0 -> 8192 / (F / fscrunch)
+                                   (FBFUSE_CB_NSAMPLES_PER_HEAP, fbfuse_constants.hpp:19-25) */
+    int quant_mode;             /* enum fbf_quant_mode */
+} fbf_cb_desc;
+
+typedef struct fbf_cb_plan fbf_cb_plan;
+
+/* offsets/scales: host arrays of F / fscrunch floats (per output channel). */
+int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets, const float* scales,
+                       fbf_cb_plan** plan);
+int fbf_cb_plan_set_levels(fbf_cb_plan* plan, const float* offsets, const float* scales);
+int fbf_cb_plan_destroy(fbf_cb_plan* plan);
+/* Kernel tile variant (0 = autotuned default); see fbf_cb_variant_count. */
+int fbf_cb_plan_set_variant(fbf_cb_plan* plan, int variant);
+int fbf_cb_variant_count(int* count);
+int fbf_cb_output_bytes(const fbf_cb_plan* plan, size_t ftpa_bytes, size_t* tbtf_bytes);
+/* FTPA + FBA -> int8 TBTF. */
+int fbf_cb_beamform(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
+                    const void* fba_weights, size_t weights_bytes,
+                    void* tbtf, size_t tbtf_bytes, fbf_stream_t stream);
+/* Exact int32 beam sums, int32 [F][T][P][B][2] (re, im). */
+int fbf_cb_beam_sums(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
+                     const void* fba_weights, size_t weights_bytes,
+                     int32_t* sums, size_t sums_bytes, fbf_stream_t stream);
+/* Exact detected power before quantisation, double [F/fs][T/ts][B]. */
+int fbf_cb_beam_power(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
+                      const void* fba_weights, size_t weights_bytes,
+                      double* power, size_t power_bytes, fbf_stream_t stream);
+
+/* ---- incoherent beamformer (IncoherentBeamformer.cu:10-117) ------------ */
+typedef struct fbf_ib_desc {
+    int nantennas;              /* all antennas of the TAFTP input */
+    int nchans;
+    int nsamples_per_heap;      /* 256 */
+    int tscrunch;               /* divides nsamples_per_heap */
+    int fscrunch;
+    int quant_mode;
+} fbf_ib_desc;
+
+typedef struct fbf_ib_plan fbf_ib_plan;
+
+int fbf_ib_plan_create(const fbf_ib_desc* desc, const float* offsets, const float* scales,
+                       fbf_ib_plan** plan);
+int fbf_ib_plan_set_levels(fbf_ib_plan* plan, const float* offsets, const float* scales);
+int fbf_ib_plan_destroy(fbf_ib_plan* plan);
+int fbf_ib_output_bytes(const fbf_ib_plan* plan, size_t taftp_bytes, size_t* tf_bytes);
+int fbf_ib_beamform(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes,
+                    void* tf, size_t tf_bytes, fbf_stream_t stream);
+/* Exact power, double [T/ts][F/fs]. */
+int fbf_ib_power(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes,
+                 double* power, size_t power_bytes, fbf_stream_t stream);
+
+/* ---- delay -> weights (WeightsManager.cu:14-123) ----------------------- */
+typedef struct fbf_weights_plan fbf_weights_plan;
+
+/* chan_freqs: host array of nchans channel centres in Hz (double). */
+int fbf_weights_plan_create(int nantennas, int nbeams, int nchans, const double* chan_freqs,
+                            fbf_weights_plan** plan);
+int fbf_weights_plan_destroy(fbf_weights_plan* plan);
+/* delays (device, [B][A] float2) at time t (s, FP64) -> weights [F][B][A] char2.
+ * phase (optional device pointer, float [F][B][A]) receives the reduced
+ * phase argument in cycles, phase - rint(phase), as passed to sin/cos. */
+int fbf_weights_generate(fbf_weights_plan* plan, const void* delays, size_t delays_bytes,
+                         double epoch, void* weights, size_t weights_bytes,
+                         float* phase, fbf_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FBFUSE_HIP_H */

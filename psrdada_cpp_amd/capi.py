@@ -1,0 +1,405 @@
+"""ctypes binding of the C ABI in ``include/fbfuse_hip.h``.
+
+This is how Python (tests, bench, smoke) reaches the native library.  The
+library is loaded from this package directory; if it is missing the import of
+any compute entry point raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libfbfuse_hip.so")
+
+FBF_OK = 0
+FBF_ERR_ARG = 1
+FBF_ERR_SIZE = 2
+FBF_ERR_HIP = 3
+FBF_ERR_UNSUPPORTED = 4
+FBF_ERR_STATE = 5
+FBF_ERR_CAPACITY = 6
+
+QUANT_ROUND = 0
+QUANT_TRUNC = 1
+
+
+class FbfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[fbf status {code}] {msg}")
+        self.code = code
+
+
+class CbDesc(C.Structure):
+    _fields_ = [("nantennas", C.c_int), ("nbeams", C.c_int), ("nchans", C.c_int),
+                ("tscrunch", C.c_int), ("fscrunch", C.c_int),
+                ("nsamples_per_heap_out", C.c_int), ("quant_mode", C.c_int)]
+
+
+class IbDesc(C.Structure):
+    _fields_ = [("nantennas", C.c_int), ("nchans", C.c_int), ("nsamples_per_heap", C.c_int),
+                ("tscrunch", C.c_int), ("fscrunch", C.c_int), ("quant_mode", C.c_int)]
+
+
+vp = C.c_void_p
+sz = C.c_size_t
+i32 = C.c_int
+pvp = C.POINTER(C.c_void_p)
+
+# name -> argtypes; every function returns int except fbf_last_error.
+_SIGS = {
+    "fbf_version": [],
+    "fbf_device_count": [C.POINTER(C.c_int)],
+    "fbf_set_device": [i32],
+    "fbf_device_name": [C.c_char_p, sz],
+    "fbf_malloc": [pvp, sz],
+    "fbf_free": [vp],
+    "fbf_host_malloc": [pvp, sz],
+    "fbf_host_free": [vp],
+    "fbf_memcpy_h2d": [vp, vp, sz, vp],
+    "fbf_memcpy_d2h": [vp, vp, sz, vp],
+    "fbf_memcpy_d2d": [vp, vp, sz, vp],
+    "fbf_memset": [vp, i32, sz, vp],
+    "fbf_stream_create": [pvp],
+    "fbf_stream_destroy": [vp],
+    "fbf_stream_synchronize": [vp],
+    "fbf_device_synchronize": [],
+    "fbf_event_create": [pvp],
+    "fbf_event_destroy": [vp],
+    "fbf_event_record": [vp, vp],
+    "fbf_event_synchronize": [vp],
+    "fbf_event_elapsed_ms": [vp, vp, C.POINTER(C.c_float)],
+    "fbf_synth_gaussian_int8": [C.c_uint64, C.c_double, i32, vp, sz],
+    "fbf_synth_uniform_f32": [C.c_uint64, C.c_float, C.c_float, vp, sz],
+    "fbf_power_levels": [C.c_float, C.c_float, i32, i32, i32, i32, i32, i32, i32,
+                         C.POINTER(C.c_float), C.POINTER(C.c_float),
+                         C.POINTER(C.c_float), C.POINTER(C.c_float)],
+    "fbf_channel_frequencies": [C.c_float, C.c_float, i32, vp],
+    "fbf_channel_frequencies_f64": [C.c_double, C.c_double, i32, vp],
+    "fbf_split_transpose": [vp, sz, vp, sz, i32, i32, i32, i32, i32, vp],
+    "fbf_cb_plan_create": [C.POINTER(CbDesc), vp, vp, pvp],
+    "fbf_cb_plan_set_levels": [vp, vp, vp],
+    "fbf_cb_plan_destroy": [vp],
+    "fbf_cb_plan_set_variant": [vp, i32],
+    "fbf_cb_variant_count": [C.POINTER(C.c_int)],
+    "fbf_cb_output_bytes": [vp, sz, C.POINTER(C.c_size_t)],
+    "fbf_cb_beamform": [vp, vp, sz, vp, sz, vp, sz, vp],
+    "fbf_cb_beam_sums": [vp, vp, sz, vp, sz, vp, sz, vp],
+    "fbf_cb_beam_power": [vp, vp, sz, vp, sz, vp, sz, vp],
+    "fbf_ib_plan_create": [C.POINTER(IbDesc), vp, vp, pvp],
+    "fbf_ib_plan_set_levels": [vp, vp, vp],
+    "fbf_ib_plan_destroy": [vp],
+    "fbf_ib_output_bytes": [vp, sz, C.POINTER(C.c_size_t)],
+    "fbf_ib_beamform": [vp, vp, sz, vp, sz, vp],
+    "fbf_ib_power": [vp, vp, sz, vp, sz, vp],
+    "fbf_weights_plan_create": [i32, i32, i32, vp, pvp],
+    "fbf_weights_plan_destroy": [vp],
+    "fbf_weights_generate": [vp, vp, sz, C.c_double, vp, sz, vp, vp],
+}
+
+_lib = None
+
+
+def header_symbols():
+    """Names of every function declared in include/*.h (parsed from the text)."""
+    import re
+    inc = os.path.join(os.path.dirname(PKG), "include")
+    names = []
+    for fn in sorted(os.listdir(inc)):
+        if fn.endswith(".h"):
+            with open(os.path.join(inc, fn)) as f:
+                txt = f.read()
+            names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(fbf_[a-z0-9_]+)\s*\(", txt, re.M)
+    return names
+
+
+def lib():
+    """Load libfbfuse_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"native library {LIB_PATH} is missing: run __graft_entry__.build(); "
+                           "there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    L.fbf_last_error.restype = C.c_char_p
+    L.fbf_last_error.argtypes = []
+    for name, args in _SIGS.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = C.c_int
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != FBF_OK:
+        msg = lib().fbf_last_error()
+        raise FbfError(status, msg.decode() if msg else "")
+    return status
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# Small RAII helpers
+# ---------------------------------------------------------------------------
+class Stream:
+    def __init__(self):
+        h = C.c_void_p()
+        call("fbf_stream_create", C.byref(h))
+        self.handle = h
+
+    def sync(self):
+        call("fbf_stream_synchronize", self.handle)
+
+    def close(self):
+        if self.handle:
+            call("fbf_stream_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        h = C.c_void_p()
+        call("fbf_event_create", C.byref(h))
+        self.handle = h
+
+    def record(self, stream):
+        call("fbf_event_record", self.handle, stream.handle if stream else None)
+
+    def sync(self):
+        call("fbf_event_synchronize", self.handle)
+
+    def elapsed_ms(self, later):
+        ms = C.c_float()
+        call("fbf_event_elapsed_ms", self.handle, later.handle, C.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                call("fbf_event_destroy", self.handle)
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Device allocation of `nbytes` bytes (hipMalloc through the C ABI)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        h = C.c_void_p()
+        call("fbf_malloc", C.byref(h), self.nbytes)
+        self.ptr = h
+
+    @classmethod
+    def from_array(cls, arr, stream=None):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        b.upload(arr, stream)
+        return b
+
+    def upload(self, arr, stream=None):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        s = stream.handle if stream else None
+        call("fbf_memcpy_h2d", self.ptr, _np_ptr(arr), arr.nbytes, s)
+        if stream:
+            stream.sync()
+        else:
+            call("fbf_device_synchronize")
+
+    def download(self, dtype=np.int8, count=None, stream=None):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = self.nbytes // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        s = stream.handle if stream else None
+        call("fbf_memcpy_d2h", _np_ptr(out), self.ptr, out.nbytes, s)
+        if stream:
+            stream.sync()
+        else:
+            call("fbf_device_synchronize")
+        return out
+
+    def fill(self, value=0, stream=None):
+        s = stream.handle if stream else None
+        call("fbf_memset", self.ptr, int(value), self.nbytes, s)
+
+    def free(self):
+        if self.ptr:
+            call("fbf_free", self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Host helpers that call native code
+# ---------------------------------------------------------------------------
+def synth_gaussian_int8(seed, sigma, n, clip=True):
+    out = np.empty(int(n), dtype=np.int8)
+    call("fbf_synth_gaussian_int8", C.c_uint64(seed), float(sigma), int(bool(clip)), _np_ptr(out), out.size)
+    return out
+
+
+def synth_uniform_f32(seed, lo, hi, n):
+    out = np.empty(int(n), dtype=np.float32)
+    call("fbf_synth_uniform_f32", C.c_uint64(seed), float(lo), float(hi), _np_ptr(out), out.size)
+    return out
+
+
+def power_levels(input_level, output_level, cb_nantennas, cb_tscrunch, cb_fscrunch,
+                 ib_nantennas, ib_tscrunch, ib_fscrunch, npol=2):
+    v = [C.c_float() for _ in range(4)]
+    call("fbf_power_levels", input_level, output_level, cb_nantennas, cb_tscrunch, cb_fscrunch,
+         ib_nantennas, ib_tscrunch, ib_fscrunch, npol, *[C.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def channel_frequencies(cfreq, bw, nchans):
+    out = np.empty(nchans, dtype=np.float32)
+    call("fbf_channel_frequencies", cfreq, bw, nchans, _np_ptr(out))
+    return out
+
+
+def channel_frequencies_f64(cfreq, bw, nchans):
+    out = np.empty(nchans, dtype=np.float64)
+    call("fbf_channel_frequencies_f64", cfreq, bw, nchans, _np_ptr(out))
+    return out
+
+
+def _s(stream):
+    return stream.handle if stream else None
+
+
+def split_transpose(taftp, ftpa, total_nantennas, used_nantennas, start_antenna, nchans,
+                    nsamples_per_heap=256, stream=None):
+    call("fbf_split_transpose", taftp.ptr, taftp.nbytes, ftpa.ptr, ftpa.nbytes, total_nantennas,
+         used_nantennas, start_antenna, nchans, nsamples_per_heap, _s(stream))
+
+
+class CbPlan:
+    """Coherent beamformer plan (fbf_cb_plan)."""
+
+    def __init__(self, nantennas, nbeams, nchans, tscrunch, fscrunch, offsets, scales,
+                 nsamples_per_heap_out=0, quant_mode=QUANT_ROUND):
+        self.desc = CbDesc(nantennas, nbeams, nchans, tscrunch, fscrunch, nsamples_per_heap_out, quant_mode)
+        off = np.ascontiguousarray(offsets, dtype=np.float32)
+        sc = np.ascontiguousarray(scales, dtype=np.float32)
+        h = C.c_void_p()
+        call("fbf_cb_plan_create", C.byref(self.desc), _np_ptr(off), _np_ptr(sc), C.byref(h))
+        self.handle = h
+
+    def set_variant(self, v):
+        call("fbf_cb_plan_set_variant", self.handle, int(v))
+
+    def output_bytes(self, ftpa_bytes):
+        n = C.c_size_t()
+        call("fbf_cb_output_bytes", self.handle, ftpa_bytes, C.byref(n))
+        return n.value
+
+    def beamform(self, ftpa, weights, out, stream=None):
+        call("fbf_cb_beamform", self.handle, ftpa.ptr, ftpa.nbytes, weights.ptr, weights.nbytes,
+             out.ptr, out.nbytes, _s(stream))
+
+    def beam_sums(self, ftpa, weights, out, stream=None):
+        call("fbf_cb_beam_sums", self.handle, ftpa.ptr, ftpa.nbytes, weights.ptr, weights.nbytes,
+             out.ptr, out.nbytes, _s(stream))
+
+    def beam_power(self, ftpa, weights, out, stream=None):
+        call("fbf_cb_beam_power", self.handle, ftpa.ptr, ftpa.nbytes, weights.ptr, weights.nbytes,
+             out.ptr, out.nbytes, _s(stream))
+
+    def close(self):
+        if self.handle:
+            call("fbf_cb_plan_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class IbPlan:
+    """Incoherent beamformer plan (fbf_ib_plan)."""
+
+    def __init__(self, nantennas, nchans, tscrunch, fscrunch, offsets, scales,
+                 nsamples_per_heap=256, quant_mode=QUANT_ROUND):
+        self.desc = IbDesc(nantennas, nchans, nsamples_per_heap, tscrunch, fscrunch, quant_mode)
+        off = np.ascontiguousarray(offsets, dtype=np.float32)
+        sc = np.ascontiguousarray(scales, dtype=np.float32)
+        h = C.c_void_p()
+        call("fbf_ib_plan_create", C.byref(self.desc), _np_ptr(off), _np_ptr(sc), C.byref(h))
+        self.handle = h
+
+    def output_bytes(self, taftp_bytes):
+        n = C.c_size_t()
+        call("fbf_ib_output_bytes", self.handle, taftp_bytes, C.byref(n))
+        return n.value
+
+    def beamform(self, taftp, out, stream=None):
+        call("fbf_ib_beamform", self.handle, taftp.ptr, taftp.nbytes, out.ptr, out.nbytes, _s(stream))
+
+    def power(self, taftp, out, stream=None):
+        call("fbf_ib_power", self.handle, taftp.ptr, taftp.nbytes, out.ptr, out.nbytes, _s(stream))
+
+    def close(self):
+        if self.handle:
+            call("fbf_ib_plan_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WeightsPlan:
+    """Delay -> weights generator plan (fbf_weights_plan)."""
+
+    def __init__(self, nantennas, nbeams, nchans, chan_freqs):
+        fr = np.ascontiguousarray(chan_freqs, dtype=np.float64)
+        assert fr.size == nchans
+        h = C.c_void_p()
+        call("fbf_weights_plan_create", nantennas, nbeams, nchans, _np_ptr(fr), C.byref(h))
+        self.handle = h
+
+    def generate(self, delays, epoch, weights, phase=None, stream=None):
+        call("fbf_weights_generate", self.handle, delays.ptr, delays.nbytes, float(epoch),
+             weights.ptr, weights.nbytes, phase.ptr if phase is not None else None, _s(stream))
+
+    def close(self):
+        if self.handle:
+            call("fbf_weights_plan_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,109 @@
+// Delay model -> int8 beamforming weights.
+//
+// Behaviour follows WeightsManager.cu:14-123 (generate_weights_k and
+// WeightsManager::weights): for delay model (rate, offset) of (beam, antenna)
+// and channel frequency f,
+//   phase = (t * rate + offset) * f          [cycles]
+//   w     = round(127 * exp(i * 2 pi * phase))   -> char2 (re = cos, im = sin)
+// written [F][B][A] (the reference's [t][F][B][A] with one time step).
+//
+// MI355X design: the phase argument is formed in FP64 -- the only FP64 step --
+// because f * tau reaches thousands of cycles at GHz x microseconds and FP32
+// would leave ~1e-3 rad of error.  The whole-cycle part is removed in FP64
+// (phase - rint(phase), exact), the reduced argument in [-1/2, 1/2] goes to
+// FP32 sincospif (HIP's device math library), then rounds half away from
+// zero to int8.  One thread per weight, antennas fastest: reads of the
+// delays and writes of the weights are coalesced.
+#include "fbf_internal.hpp"
+
+namespace fbf {
+namespace {
+
+__global__ __launch_bounds__(256) void weights_k(const float2* __restrict__ delays,
+                                                 const double* __restrict__ freqs, double epoch, int A, int B,
+                                                 long long total, char2* __restrict__ weights,
+                                                 float* __restrict__ phase_out)
+{
+#pragma clang fp contract(off)
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const long long ab = (long long)A * B;
+    const int f = (int)(idx / ab);
+    const long long di = idx - (long long)f * ab;  // b * A + a
+    const float2 d = delays[di];
+    double tau = (double)d.x * epoch;
+    tau = tau + (double)d.y;
+    const double phase = tau * freqs[f];
+    const double frac = phase - rint(phase);
+    const float x = (float)frac;
+    float s, c;
+    sincospif(2.0f * x, &s, &c);
+    const float re = 127.0f * c;
+    const float im = 127.0f * s;
+    char2 w;
+    w.x = (signed char)(int)roundf(re);
+    w.y = (signed char)(int)roundf(im);
+    weights[idx] = w;
+    if (phase_out) phase_out[idx] = x;
+}
+
+}  // namespace
+}  // namespace fbf
+
+using namespace fbf;
+
+struct fbf_weights_plan {
+    int A, B, F;
+    double* d_freqs;
+};
+
+extern "C" int fbf_weights_plan_create(int nantennas, int nbeams, int nchans, const double* chan_freqs,
+                                       fbf_weights_plan** plan)
+{
+    FBF_REQUIRE(plan, FBF_ERR_ARG, "fbf_weights_plan_create: null plan pointer");
+    *plan = nullptr;
+    FBF_REQUIRE(nantennas > 0 && nbeams > 0 && nchans > 0 && chan_freqs, FBF_ERR_ARG,
+                "fbf_weights_plan_create: bad geometry or null frequencies");
+    auto* p = new fbf_weights_plan{nantennas, nbeams, nchans, nullptr};
+    hipError_t e = hipMalloc(&p->d_freqs, sizeof(double) * nchans);
+    if (e == hipSuccess) e = hipMemcpy(p->d_freqs, chan_freqs, sizeof(double) * nchans, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (p->d_freqs) hipFree(p->d_freqs);
+        delete p;
+        return hip_status(e, "fbf_weights_plan_create");
+    }
+    *plan = p;
+    return FBF_OK;
+}
+
+extern "C" int fbf_weights_plan_destroy(fbf_weights_plan* plan)
+{
+    if (!plan) return FBF_OK;
+    hipError_t e = hipFree(plan->d_freqs);
+    delete plan;
+    if (e != hipSuccess) return hip_status(e, "hipFree(freqs)");
+    return FBF_OK;
+}
+
+extern "C" int fbf_weights_generate(fbf_weights_plan* plan, const void* delays, size_t delays_bytes, double epoch,
+                                    void* weights, size_t weights_bytes, float* phase, fbf_stream_t stream)
+{
+    FBF_REQUIRE(plan && delays && weights, FBF_ERR_ARG, "fbf_weights_generate: null argument");
+    const size_t nd = (size_t)plan->A * plan->B;
+    FBF_REQUIRE(delays_bytes == nd * sizeof(float2), FBF_ERR_SIZE,
+                "fbf_weights_generate: delays size " + std::to_string(delays_bytes) + " != expected " +
+                    std::to_string(nd * sizeof(float2)));
+    const long long total = (long long)nd * plan->F;
+    FBF_REQUIRE(weights_bytes == (size_t)total * 2, FBF_ERR_SIZE,
+                "fbf_weights_generate: weights size " + std::to_string(weights_bytes) + " != expected " +
+                    std::to_string(total * 2));
+    FBF_REQUIRE((reinterpret_cast<uintptr_t>(delays) & 7) == 0 && (reinterpret_cast<uintptr_t>(weights) & 1) == 0,
+                FBF_ERR_ARG, "fbf_weights_generate: misaligned buffer");
+    const long long nblocks = (total + 255) / 256;
+    FBF_REQUIRE(nblocks < (1ll << 31), FBF_ERR_UNSUPPORTED, "fbf_weights_generate: grid too large");
+    weights_k<<<dim3((unsigned)nblocks), 256, 0, static_cast<hipStream_t>(stream)>>>(
+        static_cast<const float2*>(delays), plan->d_freqs, epoch, plan->A, plan->B, total,
+        static_cast<char2*>(weights), phase);
+    FBF_LAUNCH_CHECK("weights_k");
+    return FBF_OK;
+}

@@ -1,0 +1,288 @@
+"""Parity of the HIP kernels (through the C ABI) against the CPU oracle.
+
+Bit-exact for split-transpose, int32 beam sums, exact power and the int8
+outputs (both the kernels and the oracle evaluate the quantiser on the exact
+power in IEEE double).  Weights: FP32 sin/cos on the GPU vs the FP64 oracle,
+tolerance stated in the tests.
+"""
+import numpy as np
+import pytest
+
+from conftest import gaussian_int8, unit_weights
+import refmath
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(capi, arr):
+    return capi.DeviceBuffer.from_array(arr)
+
+
+# ---------------------------------------------------------------------------
+# split transpose
+# ---------------------------------------------------------------------------
+def _cycling_prime(n):
+    i = np.arange(n)
+    t = np.empty((n, 2), np.int8)
+    t[:, 0] = (i % 113).astype(np.int8)
+    t[:, 1] = (i % 107).astype(np.int8)
+    return t.reshape(-1)
+
+
+@pytest.mark.parametrize("A,used,a0,F,nhg", [
+    (64, 44, 0, 64, 12),      # reference geometry (fbfuse_constants.hpp:4-13), SplitTransposeTester.cu:96-113
+    (64, 64, 0, 64, 4),
+    (64, 40, 0, 64, 2), (64, 16, 40, 64, 2), (64, 8, 56, 64, 2),   # 40/16/8 split
+    (64, 6, 3, 8, 2), (16, 3, 5, 4, 2), (128, 100, 20, 4, 1),
+])
+def test_split_transpose_bit_exact(gpu, oracle, A, used, a0, F, nhg):
+    capi = gpu
+    n = nhg * A * F * 256 * 2
+    taftp = _cycling_prime(n)
+    d_in = _dev(capi, taftp)
+    d_out = capi.DeviceBuffer(n // A * used * 2)
+    capi.split_transpose(d_in, d_out, A, used, a0, F)
+    got = d_out.download()
+    np.testing.assert_array_equal(got, oracle.split_transpose(taftp, nhg, A, used, a0, F))
+
+
+# ---------------------------------------------------------------------------
+# coherent beamformer
+# ---------------------------------------------------------------------------
+def _levels(oracle, A, ts, fs, F, level_in=16.0, level_out=24.0):
+    cbo, cbs, _, _ = oracle.power_levels(level_in, level_out, A, ts, fs, 64, 16, 1)
+    Fo = F // fs
+    return np.full(Fo, cbo, np.float32), np.full(Fo, cbs, np.float32)
+
+
+CB_SUMS_CASES = [
+    # F, T, A, B
+    (16, 2048, 16, 32),     # config 1 geometry
+    (4, 512, 44, 64),       # 44 antennas: K = 88, partial K slice
+    (2, 256, 64, 512),      # config 2 antennas/beams
+    (2, 256, 40, 768), (2, 256, 16, 256), (2, 256, 8, 64),   # config 3 subarrays
+    (1, 256, 128, 24), (3, 128, 4, 8), (1, 64, 32, 40),
+]
+
+
+@pytest.mark.parametrize("F,T,A,B", CB_SUMS_CASES)
+def test_cb_int32_sums_bit_exact(gpu, oracle, F, T, A, B):
+    capi = gpu
+    ftpa = gaussian_int8(100 + A, F * T * 2 * A * 2, 16.0)
+    w = unit_weights(200 + B, F * B * A)
+    off, sc = _levels(oracle, A, 16, 1, F)
+    plan = capi.CbPlan(A, B, F, 16 if T % 16 == 0 else 1, 1, off, sc)
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    d_s = capi.DeviceBuffer(F * T * 2 * B * 2 * 4)
+    plan.beam_sums(d_v, d_w, d_s)
+    got = d_s.download(np.int32)
+    ref = oracle.cb_sums(ftpa, w, F, T, A, B)
+    np.testing.assert_array_equal(got, ref.reshape(-1))
+
+
+@pytest.mark.parametrize("F,T,A,B,ts,fs", [
+    (16, 2048, 16, 32, 16, 1), (4, 1024, 64, 512, 16, 1), (8, 512, 44, 128, 16, 4),
+    (4, 1024, 40, 768, 64, 1), (4, 1024, 16, 256, 4, 2), (4, 512, 8, 64, 1, 1), (2, 256, 128, 16, 8, 2),
+])
+def test_cb_power_exact(gpu, oracle, F, T, A, B, ts, fs):
+    capi = gpu
+    ftpa = gaussian_int8(300 + A, F * T * 2 * A * 2, 16.0)
+    w = unit_weights(400 + B, F * B * A)
+    off, sc = _levels(oracle, A, ts, fs, F)
+    plan = capi.CbPlan(A, B, F, ts, fs, off, sc)
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    d_p = capi.DeviceBuffer((F // fs) * (T // ts) * B * 8)
+    plan.beam_power(d_v, d_w, d_p)
+    got = d_p.download(np.float64).reshape(F // fs, T // ts, B)
+    np.testing.assert_array_equal(got, oracle.cb_power(ftpa, w, F, T, A, B, ts, fs))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("F,T,A,B,ts,fs,ohs", [
+    (16, 2048, 16, 32, 16, 1, 128),         # config 1 (2048-byte output heaps)
+    (8, 2048, 64, 512, 16, 1, 0),           # config 2 geometry, 8 channels, default heap (1024 samples)
+    (8, 1024, 44, 64, 16, 4, 64),
+    (4, 4096, 40, 768, 64, 1, 0), (4, 1024, 8, 64, 4, 1, 0),
+])
+def test_cb_int8_tbtf_bit_exact(gpu, oracle, mode, F, T, A, B, ts, fs, ohs):
+    capi = gpu
+    ftpa = gaussian_int8(500 + A, F * T * 2 * A * 2, 16.0)
+    w = unit_weights(600 + B, F * B * A)
+    off, sc = _levels(oracle, A, ts, fs, F)
+    Fo, To = F // fs, T // ts
+    ohs_eff = ohs if ohs else 8192 // Fo
+    if To % ohs_eff:
+        ohs_eff = To
+        ohs = To
+    plan = capi.CbPlan(A, B, F, ts, fs, off, sc, nsamples_per_heap_out=ohs, quant_mode=mode)
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    nout = plan.output_bytes(ftpa.nbytes)
+    assert nout == To * B * Fo
+    d_o = capi.DeviceBuffer(nout)
+    plan.beamform(d_v, d_w, d_o)
+    got = d_o.download()
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, fs)
+    ref = oracle.cb_tbtf(p, off, sc, ohs_eff, mode)
+    np.testing.assert_array_equal(got, ref)
+    # outputs are not degenerate: the levels put the power inside the int8 range
+    assert got.astype(float).std() > 3.0 and np.mean(np.abs(got.astype(int)) == 127) < 0.01
+
+
+def test_cb_matches_reference_f32_within_reference_tolerance(gpu, oracle):
+    """Reference test geometry (44 antennas, ts 16, fs 4, truncating quantiser,
+    input/output level 32, unclipped N(0, 32) voltages; CoherentBeamformerTester.cu:148-185)
+    at 8 channels and 256 beams: every output within +-1 of the reference's FP32 model."""
+    capi = gpu
+    F, T, A, B, ts, fs = 8, 1024, 44, 256, 16, 4
+    ftpa = gaussian_int8(700, F * T * 2 * A * 2, 32.0, clip=False)
+    w = unit_weights(701, F * B * A)
+    cbo, cbs, _, _ = oracle.power_levels(32.0, 32.0, A, ts, fs, 64, 16, 2)
+    Fo, To = F // fs, T // ts
+    plan = capi.CbPlan(A, B, F, ts, fs, np.full(Fo, cbo, np.float32), np.full(Fo, cbs, np.float32),
+                       nsamples_per_heap_out=To, quant_mode=1)
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    d_o = capi.DeviceBuffer(To * B * Fo)
+    plan.beamform(d_v, d_w, d_o)
+    got = d_o.download()
+    ref = oracle.cb_reference_f32(ftpa, w, F, ts, fs, T, B, A, cbs, cbo, To)
+    d = np.abs(got.astype(int) - ref.astype(int))
+    assert d.max() <= 1
+    assert (d > 0).mean() < 0.01
+
+
+def test_cb_minus128_weights_follow_documented_saturation(gpu, oracle):
+    capi = gpu
+    F, T, A, B = 2, 64, 16, 16
+    ftpa = gaussian_int8(800, F * T * 2 * A * 2, 30.0, clip=False)
+    w = unit_weights(801, F * B * A).reshape(F, B, A, 2).copy()
+    w[0, 3, 5, 1] = -128
+    w[1, 7, 0, 1] = -128
+    w[1, 7, 1, 0] = -128
+    w = w.reshape(-1)
+    off, sc = _levels(oracle, A, 16, 1, F)
+    plan = capi.CbPlan(A, B, F, 16, 1, off, sc)
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    d_s = capi.DeviceBuffer(F * T * 2 * B * 2 * 4)
+    plan.beam_sums(d_v, d_w, d_s)
+    np.testing.assert_array_equal(d_s.download(np.int32), oracle.cb_sums(ftpa, w, F, T, A, B).reshape(-1))
+
+
+def test_cb_unit_weights_closed_form(gpu):
+    """w = (127, 0): beam = 127 * sum_a v (computed here with numpy)."""
+    capi = gpu
+    F, T, A, B = 2, 256, 64, 64
+    ftpa = gaussian_int8(900, F * T * 2 * A * 2, 16.0)
+    w = np.zeros(F * B * A * 2, np.int8)
+    w[0::2] = 127
+    plan = capi.CbPlan(A, B, F, 16, 1, np.ones(F, np.float32), np.ones(F, np.float32))
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    d_s = capi.DeviceBuffer(F * T * 2 * B * 2 * 4)
+    plan.beam_sums(d_v, d_w, d_s)
+    s = d_s.download(np.int32).reshape(F, T, 2, B, 2)
+    v = ftpa.reshape(F, T, 2, A, 2).astype(np.int64).sum(axis=3)
+    for b in (0, 17, 63):
+        np.testing.assert_array_equal(s[:, :, :, b, :], 127 * v)
+
+
+def test_cb_rejects_bad_sizes(gpu):
+    capi = gpu
+    plan = capi.CbPlan(64, 512, 4, 16, 1, np.ones(4, np.float32), np.ones(4, np.float32))
+    d_v = capi.DeviceBuffer(4 * 256 * 2 * 64 * 2)
+    d_w = capi.DeviceBuffer(4 * 512 * 64 * 2 - 2)
+    d_o = capi.DeviceBuffer(16 * 512 * 4)
+    with pytest.raises(capi.FbfError) as e:
+        plan.beamform(d_v, d_w, d_o)
+    assert e.value.code == capi.FBF_ERR_SIZE
+
+
+# ---------------------------------------------------------------------------
+# incoherent beamformer
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("A,F,nhg,ts,fs", [
+    (64, 64, 4, 16, 1), (64, 64, 2, 16, 2), (64, 64, 2, 1, 1), (64, 64, 2, 2, 4),
+    (16, 16, 8, 16, 1), (40, 32, 2, 64, 8), (8, 8, 3, 4, 1), (64, 64, 1, 256, 16),
+])
+def test_ib_bit_exact(gpu, oracle, A, F, nhg, ts, fs):
+    capi = gpu
+    taftp = gaussian_int8(1000 + A + ts, nhg * A * F * 256 * 4, 16.0)
+    _, _, ibo, ibs = oracle.power_levels(16.0, 24.0, A, 16, 1, A, ts, fs)
+    Fo = F // fs
+    off, sc = np.full(Fo, ibo, np.float32), np.full(Fo, ibs, np.float32)
+    d_in = _dev(capi, taftp)
+    p_ref = oracle.ib_power(taftp, F, ts, fs, nhg, A)
+    for mode in (0, 1):
+        plan = capi.IbPlan(A, F, ts, fs, off, sc, quant_mode=mode)
+        d_p = capi.DeviceBuffer(p_ref.size * 8)
+        plan.power(d_in, d_p)
+        np.testing.assert_array_equal(d_p.download(np.float64).reshape(p_ref.shape), p_ref)
+        d_o = capi.DeviceBuffer(plan.output_bytes(taftp.nbytes))
+        plan.beamform(d_in, d_o)
+        np.testing.assert_array_equal(d_o.download(), oracle.ib_tf(p_ref, off, sc, mode))
+
+
+def test_ib_matches_reference_f32_within_tolerance(gpu, oracle):
+    """IncoherentBeamformerTester.cu:124-145 geometry: 64 antennas, 64 channels,
+    ts 16, fs 2, 32 heap groups, N(0, 32) voltages, levels 32/32, truncation."""
+    capi = gpu
+    A, F, nhg, ts, fs = 64, 64, 32, 16, 2
+    taftp = gaussian_int8(1100, nhg * A * F * 256 * 4, 32.0, clip=False)
+    _, _, ibo, ibs = oracle.power_levels(32.0, 32.0, 44, 16, 4, A, ts, fs)
+    Fo = F // fs
+    plan = capi.IbPlan(A, F, ts, fs, np.full(Fo, ibo, np.float32), np.full(Fo, ibs, np.float32), quant_mode=1)
+    d_in = _dev(capi, taftp)
+    d_o = capi.DeviceBuffer(plan.output_bytes(taftp.nbytes))
+    plan.beamform(d_in, d_o)
+    ref = oracle.ib_reference_f32(taftp, F, ts, fs, nhg, A, ibs, ibo)
+    assert np.abs(d_o.download().astype(int) - ref.astype(int)).max() <= 1
+
+
+# ---------------------------------------------------------------------------
+# weights
+# ---------------------------------------------------------------------------
+def _weights_gpu(capi, d, freqs, epoch, A, B, F):
+    plan = capi.WeightsPlan(A, B, F, freqs)
+    d_d = _dev(capi, d)
+    d_w = capi.DeviceBuffer(F * B * A * 2)
+    d_ph = capi.DeviceBuffer(F * B * A * 4)
+    plan.generate(d_d, epoch, d_w, d_ph)
+    return d_w.download(), d_ph.download(np.float32)
+
+
+def test_weights_zero_value(gpu, oracle):
+    """WeightsManagerTester.cu:95-106: zero delays at epoch 0 -> every weight (127, 0)."""
+    A, B, F = 44, 1024, 64
+    fr = oracle.channel_frequencies(1.4e9, 56e6, F).astype(np.float64)
+    w, _ = _weights_gpu(gpu, np.zeros(A * B * 2, np.float32), fr, 0.0, A, B, F)
+    assert np.all(w[0::2] == 127) and np.all(w[1::2] == 0)
+
+
+def test_weights_real_value(gpu, oracle):
+    """WeightsManagerTester.cu:108-119 inputs {rate 1e-11, offset 1e-10}, epoch 10."""
+    A, B, F = 44, 1024, 64
+    d = np.tile(np.array([1e-11, 1e-10], np.float32), A * B)
+    fr = oracle.channel_frequencies(1.4e9, 56e6, F).astype(np.float64)
+    w, ph = _weights_gpu(gpu, d, fr, 10.0, A, B, F)
+    ref, frac = oracle.weights_f64(d, fr, 10.0, A, B, F, want_phase=True)
+    np.testing.assert_array_equal(ph, frac.astype(np.float32))
+    diff = np.abs(w.astype(int) - ref.astype(int))
+    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+
+
+@pytest.mark.parametrize("epoch", [0.0, 75.0, 300.0, 599.9])
+def test_weights_fp64_phase_config4(gpu, oracle, epoch):
+    """64 antennas x 1024 beams x 64 channels, tau0 ~ U(-2us, 2us), rate ~ U(-1e-9, 1e-9),
+    epochs over 600 s.  Phase argument: the GPU's FP32 reduced phase equals the
+    FP64 oracle's rounded to FP32 (max error 2^-25 cycles = 1.9e-7 rad);
+    int8 weights within 1 LSB of the FP64 oracle on < 1e-3 of the components."""
+    capi = gpu
+    from psrdada_cpp_amd import workloads as wl
+    A, B, F = 64, 1024, 64
+    d = wl.synth_delays(B, A, 4242, 2e-6, rate_max=1e-9)
+    fr = capi.channel_frequencies_f64(wl.L_BAND_CENTRE_HZ, F * wl.CHANNEL_BW_HZ, F)
+    w, ph = _weights_gpu(capi, d, fr, epoch, A, B, F)
+    ref, frac = oracle.weights_f64(d, fr, epoch, A, B, F, want_phase=True)
+    err_rad = np.abs(ph.astype(np.float64) - frac).max() * 2 * np.pi
+    assert err_rad <= 2 * np.pi * 2.0 ** -25
+    diff = np.abs(w.astype(int) - ref.astype(int))
+    assert diff.max() <= 1
+    assert (diff > 0).mean() < 1e-3

@@ -1,0 +1,358 @@
+"""Pins the CPU oracle (oracle/) against known answers, closed forms and
+independent numpy formulations.  CPU only.
+
+The reference keeps no stored golden vectors (no test/data directory); its
+known-answer tests are reproduced here: the split-transpose cycling-prime
+pattern (SplitTransposeTester.cu:96-113), the zero-delay weights
+(WeightsManagerTester.cu:95-106) and the real-value weights
+(WeightsManagerTester.cu:108-119), plus the reference's own +-1 tolerance for
+the coherent and incoherent beams (CoherentBeamformerTester.cu:144,
+IncoherentBeamformerTester.cu:120).  Committed fixtures under tests/golden/
+hold small hand-checked vectors.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import gaussian_int8, rng, unit_weights
+import refmath
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------------------
+# split transpose
+# ---------------------------------------------------------------------------
+def test_split_transpose_hand_kat(oracle):
+    # 1 heap group, 3 antennas (use antennas 1..2), 2 channels, S = 32.
+    nhg, A, used, a0, F, S = 1, 3, 2, 1, 2, 32
+    n = nhg * A * F * S * 2
+    taftp = np.arange(2 * n, dtype=np.int64).astype(np.int8)   # byte i = i mod 256
+    out = oracle.split_transpose(taftp, nhg, A, used, a0, F, S)
+    # Element (f=1, s=5, p=1, a=0) of the output comes from input
+    # (hg=0, antenna 1, f=1, s=5, p=1): element index ((0*3+1)*2+1)*64 + 5*2 + 1 = 203.
+    out_el = ((1 * (nhg * S) + 5) * 2 + 1) * used + 0
+    # input bytes 406 and 407 hold (406 mod 256, 407 mod 256) = (150, 151) -> int8 (-106, -105)
+    assert out[2 * out_el] == -106
+    assert out[2 * out_el + 1] == -105
+
+
+@pytest.mark.parametrize("A,used,a0,F,nhg", [(64, 44, 0, 64, 12), (64, 40, 0, 8, 2), (64, 16, 40, 8, 2),
+                                              (64, 8, 56, 8, 2), (16, 16, 0, 16, 3), (8, 3, 5, 4, 2)])
+def test_split_transpose_cycling_prime(oracle, A, used, a0, F, nhg):
+    """SplitTransposeTester.cu:96-113 pattern (x = i % 113, y = i % 107)."""
+    S = 256
+    n = nhg * A * F * S * 2
+    i = np.arange(n)
+    taftp = np.empty((n, 2), np.int8)
+    taftp[:, 0] = (i % 113).astype(np.int8)
+    taftp[:, 1] = (i % 107).astype(np.int8)
+    taftp = taftp.reshape(-1)
+    out = oracle.split_transpose(taftp, nhg, A, used, a0, F, S)
+    ref = refmath.split_transpose_np(taftp, nhg, A, used, a0, F, S)
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_split_transpose_is_a_permutation(oracle):
+    nhg, A, F, S = 2, 8, 4, 256
+    taftp = gaussian_int8(3, nhg * A * F * S * 4)
+    out = oracle.split_transpose(taftp, nhg, A, A, 0, F, S)
+    a = np.sort(taftp.view(np.int16))
+    b = np.sort(out.view(np.int16))
+    np.testing.assert_array_equal(a, b)
+
+
+# ---------------------------------------------------------------------------
+# coherent beam sums and power
+# ---------------------------------------------------------------------------
+def _cb_case(seed, F, T, A, B):
+    ftpa = gaussian_int8(seed, F * T * 2 * A * 2, 16.0)
+    w = unit_weights(seed + 1, F * B * A)
+    return ftpa, w
+
+
+@pytest.mark.parametrize("F,T,A,B", [(2, 32, 16, 8), (3, 16, 44, 16), (1, 64, 64, 24)])
+def test_cb_sums_match_complex_matmul(oracle, F, T, A, B):
+    ftpa, w = _cb_case(11, F, T, A, B)
+    s = oracle.cb_sums(ftpa, w, F, T, A, B)
+    ref = refmath.cb_sums_np(ftpa, w, F, T, A, B)
+    np.testing.assert_array_equal(s.astype(np.int64), ref)
+
+
+def test_cb_sums_closed_form_unit_weights(oracle):
+    """w = (127, 0) for all -> beam = 127 * sum_a v."""
+    F, T, A, B = 2, 16, 8, 8
+    ftpa, _ = _cb_case(5, F, T, A, B)
+    w = np.zeros(F * B * A * 2, np.int8)
+    w[0::2] = 127
+    s = oracle.cb_sums(ftpa, w, F, T, A, B)
+    v = ftpa.reshape(F, T, 2, A, 2).astype(np.int64).sum(axis=3)   # F T P 2
+    for b in range(B):
+        np.testing.assert_array_equal(s[:, :, :, b, :], 127 * v)
+
+
+def test_cb_sums_closed_form_quarter_turns(oracle):
+    """w_a in {(127,0),(0,127),(-127,0),(0,-127)}, v_a = c * conj(w_a)/127 -> beam = 127 c A."""
+    F, T, A, B = 1, 4, 16, 8
+    r = rng(7)
+    quarter = np.array([[127, 0], [0, 127], [-127, 0], [0, -127]], np.int64)
+    k = r.integers(0, 4, size=(B, A))
+    w = quarter[k]                                   # B A 2
+    c = np.array([3 + 2j, -5 + 1j, 7 - 4j, 2 + 0j])  # per time sample
+    ftpa = np.zeros((F, T, 2, A, 2), np.int64)
+    b_sel = 3
+    for t in range(T):
+        wc = w[b_sel, :, 0] - 1j * w[b_sel, :, 1]    # conj(w)
+        v = c[t] * wc / 127
+        ftpa[0, t, :, :, 0] = np.rint(v.real)
+        ftpa[0, t, :, :, 1] = np.rint(v.imag)
+    s = oracle.cb_sums(ftpa.astype(np.int8).reshape(-1), w.astype(np.int8).reshape(-1), F, T, A, B)
+    for t in range(T):
+        exp = 127 * c[t] * A
+        assert s[0, t, 0, b_sel, 0] == int(exp.real) and s[0, t, 0, b_sel, 1] == int(exp.imag)
+
+
+def test_cb_sums_single_antenna_textbook_multiply(oracle):
+    F, T, A, B = 1, 8, 4, 8
+    ftpa = np.zeros((F, T, 2, A, 2), np.int8)
+    ftpa[0, :, :, 2, 0] = 17
+    ftpa[0, :, :, 2, 1] = -9
+    w = unit_weights(3, F * B * A).reshape(F, B, A, 2)
+    s = oracle.cb_sums(ftpa.reshape(-1), w.reshape(-1), F, T, A, B)
+    for b in range(B):
+        wr, wi = int(w[0, b, 2, 0]), int(w[0, b, 2, 1])
+        assert s[0, 0, 0, b, 0] == wr * 17 - wi * (-9)
+        assert s[0, 0, 0, b, 1] == wr * (-9) + wi * 17
+
+
+def test_cb_sums_linear_in_voltages(oracle):
+    F, T, A, B = 1, 16, 8, 8
+    v1 = gaussian_int8(1, F * T * 2 * A * 2, 8.0)
+    v2 = gaussian_int8(2, F * T * 2 * A * 2, 8.0)
+    w = unit_weights(3, F * B * A)
+    s1 = oracle.cb_sums(v1, w, F, T, A, B)
+    s2 = oracle.cb_sums(v2, w, F, T, A, B)
+    s12 = oracle.cb_sums((v1.astype(np.int16) + v2).astype(np.int8), w, F, T, A, B)
+    np.testing.assert_array_equal(s12, s1 + s2)
+
+
+def test_cb_sums_negated_minus128_saturates(oracle):
+    """The Re row carries -wi; -(-128) saturates to 127 (documented semantic)."""
+    F, T, A, B = 1, 4, 4, 8
+    ftpa = gaussian_int8(9, F * T * 2 * A * 2, 20.0)
+    w = unit_weights(10, F * B * A).reshape(F, B, A, 2).copy()
+    w[0, 1, 2, 1] = -128
+    s = oracle.cb_sums(ftpa, w.reshape(-1), F, T, A, B)
+    v = ftpa.reshape(F, T, 2, A, 2).astype(np.int64)
+    wr = w[0, 1, :, 0].astype(np.int64)
+    wi = w[0, 1, :, 1].astype(np.int64)
+    nwi = -wi
+    nwi[2] = 127
+    re = (v[0, :, :, :, 0] * wr + v[0, :, :, :, 1] * nwi).sum(-1)
+    im = (v[0, :, :, :, 1] * wr + v[0, :, :, :, 0] * wi).sum(-1)
+    np.testing.assert_array_equal(s[0, :, :, 1, 0], re)
+    np.testing.assert_array_equal(s[0, :, :, 1, 1], im)
+
+
+@pytest.mark.parametrize("ts,fs", [(16, 1), (4, 2), (1, 1), (8, 4)])
+def test_cb_power_matches_numpy(oracle, ts, fs):
+    F, T, A, B = 4, 64, 16, 16
+    ftpa, w = _cb_case(21, F, T, A, B)
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, fs)
+    ref = refmath.cb_power_np(ftpa, w, F, T, A, B, ts, fs)
+    np.testing.assert_array_equal(p, ref)
+
+
+def test_cb_power_full_range_exact(oracle):
+    """Extreme inputs: |r| near 2^21 for 128 antennas -- power still exact."""
+    F, T, A, B = 1, 32, 128, 8
+    ftpa = np.full(F * T * 2 * A * 2, -128, np.int8)
+    w = np.full(F * B * A * 2, -127, np.int8)
+    p = oracle.cb_power(ftpa, w, F, T, A, B, 32, 1)
+    # beam = A * (-127 - 127 i)(-128 - 128 i) = A * 127 * 128 * 2i
+    beam = A * 127 * 128 * 2
+    assert p[0, 0, 0] == float(beam * beam * 2 * 32)
+
+
+# ---------------------------------------------------------------------------
+# quantiser
+# ---------------------------------------------------------------------------
+def test_quantise_edge_cases(oracle):
+    q = oracle.quantise
+    # round half away from zero
+    assert q(10.5, 0.0, 1.0) == 11
+    assert q(-10.5, 0.0, 1.0) == -11
+    assert q(10.4999, 0.0, 1.0) == 10
+    assert q(0.5, 0.0, 1.0) == 1 and q(-0.5, 0.0, 1.0) == -1
+    # truncation (reference cast)
+    assert q(10.9, 0.0, 1.0, 1) == 10 and q(-10.9, 0.0, 1.0, 1) == -10
+    # offset and scale
+    assert q(1000.0, 100.0, 10.0) == 90
+    # saturation
+    assert q(127.5, 0.0, 1.0) == 127 and q(128.0, 0.0, 1.0, 1) == 127
+    assert q(-128.4, 0.0, 1.0) == -128 and q(-1e9, 0.0, 1.0) == -128 and q(1e12, 0.0, 1.0) == 127
+    assert q(float("nan"), 0.0, 1.0) == -128
+
+
+def test_quantise_matches_numpy(oracle):
+    p = rng(4).uniform(-300, 300, size=2000)
+    for mode in (0, 1):
+        ref = refmath.quantise_np(p, 3.25, 1.5, mode)
+        got = np.array([oracle.quantise(x, 3.25, 1.5, mode) for x in p], np.int8)
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_cb_tbtf_layout(oracle):
+    Fo, To, B, ohs = 4, 8, 16, 4
+    pw = rng(5).uniform(0, 100, size=(Fo, To, B))
+    off = np.full(Fo, 50.0, np.float32)
+    sc = np.full(Fo, 0.5, np.float32)
+    got = oracle.cb_tbtf(pw, off, sc, ohs)
+    q = refmath.quantise_np(pw, 50.0, 0.5)
+    np.testing.assert_array_equal(got, refmath.tbtf_from_fo_to_b(q, ohs))
+
+
+def test_cb_reference_f32_agrees_within_reference_tolerance(oracle):
+    """The reference's FP32 host model vs this build's exact-power TRUNC path:
+    within the reference's own +-1 (CoherentBeamformerTester.cu:144)."""
+    F, T, A, B, ts, fs = 8, 256, 44, 32, 16, 4
+    ftpa = gaussian_int8(31, F * T * 2 * A * 2, 32.0, clip=False)
+    w = unit_weights(32, F * B * A)
+    cbo, cbs, _, _ = oracle.power_levels(32.0, 32.0, A, ts, fs, 64, 16, 2)
+    ohs = (T // ts)
+    ref = oracle.cb_reference_f32(ftpa, w, F, ts, fs, T, B, A, cbs, cbo, ohs)
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, fs)
+    exact = oracle.cb_tbtf(p, np.full(F // fs, cbo, np.float32), np.full(F // fs, cbs, np.float32), ohs, 1)
+    d = np.abs(ref.astype(int) - exact.astype(int))
+    assert d.max() <= 1
+    assert (d > 0).mean() < 0.01
+
+
+# ---------------------------------------------------------------------------
+# incoherent beam
+# ---------------------------------------------------------------------------
+def test_ib_power_closed_form_constant(oracle):
+    nhg, A, F, ts, fs = 2, 8, 4, 16, 2
+    taftp = np.tile(np.array([3, -4], np.int8), nhg * A * F * 256 * 2)
+    p = oracle.ib_power(taftp, F, ts, fs, nhg, A)
+    assert np.all(p == A * fs * ts * 2 * 25)
+
+
+@pytest.mark.parametrize("ts,fs", [(16, 1), (16, 2), (1, 1), (4, 4), (64, 8)])
+def test_ib_power_matches_numpy(oracle, ts, fs):
+    nhg, A, F = 2, 8, 8
+    taftp = gaussian_int8(41, nhg * A * F * 256 * 4, 30.0, clip=False)
+    p = oracle.ib_power(taftp, F, ts, fs, nhg, A)
+    np.testing.assert_array_equal(p, refmath.ib_power_np(taftp, nhg, A, F, ts, fs))
+
+
+def test_ib_reference_f32_agrees(oracle):
+    nhg, A, F, ts, fs = 4, 64, 64, 16, 2
+    taftp = gaussian_int8(42, nhg * A * F * 256 * 4, 32.0, clip=False)
+    _, _, ibo, ibs = oracle.power_levels(32.0, 32.0, 44, 16, 4, A, ts, fs)
+    ref = oracle.ib_reference_f32(taftp, F, ts, fs, nhg, A, ibs, ibo)
+    p = oracle.ib_power(taftp, F, ts, fs, nhg, A)
+    exact = oracle.ib_tf(p, np.full(F // fs, ibo, np.float32), np.full(F // fs, ibs, np.float32), 1)
+    assert np.abs(ref.astype(int) - exact.astype(int)).max() <= 1
+
+
+# ---------------------------------------------------------------------------
+# weights
+# ---------------------------------------------------------------------------
+def test_weights_zero_delay_is_127(oracle):
+    """WeightsManagerTester.cu:95-106: all delays 0 -> every weight (127, 0)."""
+    A, B, F = 8, 16, 64
+    d = np.zeros(A * B * 2, np.float32)
+    fr = oracle.channel_frequencies(1.4e9, 56e6, F).astype(np.float64)
+    w = oracle.weights_f64(d, fr, 0.0, A, B, F)
+    assert np.all(w[0::2] == 127) and np.all(w[1::2] == 0)
+    wr = oracle.weights_reference_f32(d, fr.astype(np.float32), 0.0, A, B, F)
+    np.testing.assert_array_equal(w, wr)
+
+
+def test_weights_real_value_matches_reference_f32(oracle):
+    """WeightsManagerTester.cu:108-119 inputs: {1e-11, 1e-10}, epoch 10, 1.4 GHz / 56 MHz."""
+    A, B, F = 44, 32, 64
+    d = np.tile(np.array([1e-11, 1e-10], np.float32), A * B)
+    fr32 = oracle.channel_frequencies(1.4e9, 56e6, F)
+    w64 = oracle.weights_f64(d, fr32.astype(np.float64), 10.0, A, B, F)
+    w32 = oracle.weights_reference_f32(d, fr32, 10.0, A, B, F)
+    assert np.abs(w64.astype(int) - w32.astype(int)).max() <= 1
+
+
+def test_weights_match_numpy_fp64(oracle):
+    A, B, F = 8, 8, 16
+    r = rng(12)
+    d = np.empty((B * A, 2), np.float32)
+    d[:, 0] = r.uniform(-1e-9, 1e-9, B * A)
+    d[:, 1] = r.uniform(-2e-6, 2e-6, B * A)
+    fr = np.linspace(856e6, 1712e6, F)
+    t = 437.25
+    w, ph = oracle.weights_f64(d.reshape(-1), fr, t, A, B, F, want_phase=True)
+    tau = d[:, 0].astype(np.float64) * t + d[:, 1].astype(np.float64)
+    phase = tau[None, :] * fr[:, None]
+    frac = phase - np.rint(phase)
+    np.testing.assert_array_equal(ph.reshape(F, B * A), frac)
+    re = np.clip(refmath.quantise_np(127 * np.cos(2 * np.pi * frac), 0.0, 1.0), -128, 127)
+    im = np.clip(refmath.quantise_np(127 * np.sin(2 * np.pi * frac), 0.0, 1.0), -128, 127)
+    ww = w.reshape(F, B * A, 2)
+    assert np.abs(ww[..., 0].astype(int) - re).max() <= 1 and np.mean(ww[..., 0] != re) < 1e-3
+    assert np.abs(ww[..., 1].astype(int) - im).max() <= 1 and np.mean(ww[..., 1] != im) < 1e-3
+
+
+def test_weights_symmetries(oracle):
+    A, B, F = 4, 4, 8
+    r = rng(13)
+    d = np.zeros((B * A, 2), np.float32)
+    d[:, 1] = r.uniform(-1e-6, 1e-6, B * A)
+    fr = np.linspace(900e6, 1700e6, F)
+    w = oracle.weights_f64(d.reshape(-1), fr, 0.0, A, B, F).reshape(-1, 2)
+    dn = d.copy()
+    dn[:, 1] = -dn[:, 1]
+    wn = oracle.weights_f64(dn.reshape(-1), fr, 0.0, A, B, F).reshape(-1, 2)
+    np.testing.assert_array_equal(wn[:, 0], w[:, 0])      # tau -> -tau conjugates
+    np.testing.assert_array_equal(wn[:, 1], -w[:, 1])
+    # f * tau an integer number of cycles -> (127, 0)
+    di = np.zeros((B * A, 2), np.float32)
+    di[:, 1] = 1e-6
+    wi = oracle.weights_f64(di.reshape(-1), np.array([1e9, 2e9, 3e9, 5e9, 7e9, 1e10, 2e10, 3e10]), 0.0,
+                            A, B, F).reshape(-1, 2)
+    assert np.all(wi[:, 0] == 127) and np.all(wi[:, 1] == 0)
+
+
+# ---------------------------------------------------------------------------
+# configuration formulas
+# ---------------------------------------------------------------------------
+def test_power_levels_closed_form(oracle):
+    cbo, cbs, ibo, ibs = oracle.power_levels(32.0, 24.0, 44, 16, 4, 64, 16, 2)
+    s = np.float32((127.0 * 32.0 * np.sqrt(44.0)) ** 2)
+    dof = 2 * 16 * 4 * 2
+    assert cbo == pytest.approx(float(s) * dof, rel=1e-6)
+    assert cbs == pytest.approx(float(s) * np.sqrt(2 * dof) / 24.0, rel=1e-6)
+    idof = 2 * 16 * 2 * 64 * 2
+    assert ibo == pytest.approx(32.0 ** 2 * idof, rel=1e-7)
+    assert ibs == pytest.approx(32.0 ** 2 * np.sqrt(2 * idof) / 24.0, rel=1e-6)
+
+
+def test_channel_frequencies_closed_form(oracle):
+    f = oracle.channel_frequencies(1.4e9, 56e6, 64)
+    exp = 1.4e9 - 28e6 + 56e6 / 64 * (np.arange(64) + 0.5)
+    np.testing.assert_allclose(f, exp, rtol=1e-7)
+
+
+# ---------------------------------------------------------------------------
+# committed fixtures
+# ---------------------------------------------------------------------------
+def test_golden_fixtures(oracle):
+    path = os.path.join(GOLDEN, "cb_small.json")
+    with open(path) as fh:
+        g = json.load(fh)
+    F, T, A, B, ts, fs = (g[k] for k in ("F", "T", "A", "B", "ts", "fs"))
+    ftpa = np.array(g["ftpa"], np.int8)
+    w = np.array(g["weights"], np.int8)
+    s = oracle.cb_sums(ftpa, w, F, T, A, B)
+    np.testing.assert_array_equal(s.reshape(-1), np.array(g["sums"], np.int64))
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, fs)
+    np.testing.assert_array_equal(p.reshape(-1), np.array(g["power"], np.float64))

@@ -22,9 +22,12 @@
 // the 16 columns of a tile are 16 different output samples: column c of the
 // j-th tile is input column (t0 + c) * tscrunch * P + j.  Detection and the
 // whole tscrunch x pol x fscrunch integration therefore happen in-lane with no
-// cross-lane traffic.  Power is accumulated in FP64, which is exact: every
-// partial sum is an integer below 2^53 (|r| < 2^22 for A <= 128).
+// cross-lane traffic.  Power is accumulated exactly in int64 with
+// v_mad_i64_i32 (one instruction per squared int32 sum; |r| < 2^22 for
+// A <= 128) and handed to the quantiser as a double, like the oracle.
 #include "fbf_internal.hpp"
+
+#include <type_traits>
 
 namespace fbf {
 namespace {
@@ -45,24 +48,38 @@ struct CbArgs {
     int n_bchunks, n_tchunks, n_bgroups, n_tgroups, wb;
 };
 
-// 16 bytes of one K slice [k0, k0 + 16) of a column of K bytes; bytes at
-// k >= K read as zero.  K is a multiple of 8.
-__device__ __forceinline__ v4i load_kslice(const int8_t* col, int k0, int K, bool al16)
+// Loads of one 16-byte K slice [k0, k0 + 16) of a column of K bytes.  They
+// are unconditional and branch-free (divergent branches around loads force
+// the compiler to wait for every load in flight): addresses are clamped
+// inside the column instead.  K is a multiple of 8; AL16 means K % 16 == 0.
+//
+// Voltage slices: bytes at k >= K may hold anything, because the weights
+// there are zero.  A slice past the column end re-reads the column start; the
+// upper half of a slice straddling K (K % 16 == 8) re-reads its lower half.
+template <bool AL16>
+__device__ __forceinline__ v4i load_vslice(const int8_t* col, int k0, int K)
 {
-    v4i r = {0, 0, 0, 0};
-    if (k0 + 16 <= K) {
-        if (al16) {
-            r = *reinterpret_cast<const v4i*>(col + k0);
-        } else {
-            const int2 lo = *reinterpret_cast<const int2*>(col + k0);
-            const int2 hi = *reinterpret_cast<const int2*>(col + k0 + 8);
-            r = v4i{lo.x, lo.y, hi.x, hi.y};
-        }
-    } else if (k0 + 8 <= K) {
-        const int2 lo = *reinterpret_cast<const int2*>(col + k0);
-        r = v4i{lo.x, lo.y, 0, 0};
+    if (AL16) {
+        const int kk = k0 < K ? k0 : 0;
+        return *reinterpret_cast<const v4i*>(col + kk);
+    } else {
+        const int klo = k0 < K ? k0 : 0;
+        const int khi = k0 + 8 < K ? k0 + 8 : klo;
+        const int2 lo = *reinterpret_cast<const int2*>(col + klo);
+        const int2 hi = *reinterpret_cast<const int2*>(col + khi);
+        return v4i{lo.x, lo.y, hi.x, hi.y};
     }
-    return r;
+}
+
+// Weight slices: bytes at k >= K, and every byte of a row past the last
+// beam (row_valid false), are zero.
+template <bool AL16>
+__device__ __forceinline__ v4i load_wslice(const int8_t* col, int k0, int K, bool row_valid)
+{
+    const v4i r = load_vslice<AL16>(col, k0, K);
+    const bool lo_ok = row_valid && k0 < K;
+    const bool hi_ok = row_valid && k0 + 8 < K;
+    return v4i{lo_ok ? r.x : 0, lo_ok ? r.y : 0, hi_ok ? r.z : 0, hi_ok ? r.w : 0};
 }
 
 // Im row: [wr, wi] -> [wi, wr] per antenna (swap bytes in each 16-bit half).
@@ -83,7 +100,7 @@ __device__ __forceinline__ uint32_t neg_odd(uint32_t x)
     return (x & 0x00FF00FFu) | (sat_neg_byte(x >> 8) << 8) | (sat_neg_byte(x >> 24) << 24);
 }
 
-template <int ROWT, int KSTEPS, int MODE>
+template <int ROWT, int KSTEPS, int MODE, bool AL16>
 __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
 {
     const int lane = threadIdx.x & 63;
@@ -101,16 +118,17 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
     if (bc >= p.n_bchunks || tc >= p.n_tchunks || fo >= p.Fo) return;
 
     const int K = 2 * p.A;
-    const bool al16 = (K & 15) == 0;
     const int beam_base = bc * (8 * ROWT);
     const long long t_out = (long long)tc * 16 + c;
     const bool t_valid = t_out < p.nsamples_out;
+    // Lanes past the last output sample read (and never store) the last one.
+    const long long t_addr = t_valid ? t_out : p.nsamples_out - 1;
     const int tsp = 2 * p.ts;
     const long long TP = 2 * p.nsamples;
 
-    double pw[ROWT][2];
+    long long pw[ROWT][2];
 #pragma unroll
-    for (int r = 0; r < ROWT; ++r) pw[r][0] = pw[r][1] = 0.0;
+    for (int r = 0; r < ROWT; ++r) pw[r][0] = pw[r][1] = 0;
 
     for (int fc = 0; fc < p.fs; ++fc) {
         const int f = fo * p.fs + fc;
@@ -119,11 +137,11 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
 #pragma unroll
         for (int r = 0; r < ROWT; ++r) {
             const int b = beam_base + r * 8 + (c >> 1);
-            const int8_t* wcol = p.fba + ((size_t)f * p.B + b) * K;
+            const bool bvalid = b < p.B;
+            const int8_t* wcol = p.fba + ((size_t)f * p.B + (bvalid ? b : p.B - 1)) * K;
 #pragma unroll
             for (int s = 0; s < KSTEPS; ++s) {
-                v4i raw = {0, 0, 0, 0};
-                if (b < p.B) raw = load_kslice(wcol, s * 64 + g * 16, K, al16);
+                const v4i raw = load_wslice<AL16>(wcol, s * 64 + g * 16, K, bvalid);
                 v4i e;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
@@ -132,20 +150,18 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
             }
         }
 
-        const int8_t* vcol = p.ftpa + ((size_t)f * TP + (size_t)t_out * tsp) * K;
+        const int8_t* vcol = p.ftpa + ((size_t)f * TP + (size_t)t_addr * tsp) * K;
         v4i v[KSTEPS];
 #pragma unroll
-        for (int s = 0; s < KSTEPS; ++s)
-            v[s] = t_valid ? load_kslice(vcol, s * 64 + g * 16, K, al16) : v4i{0, 0, 0, 0};
+        for (int s = 0; s < KSTEPS; ++s) v[s] = load_vslice<AL16>(vcol, s * 64 + g * 16, K);
 
         for (int j = 0; j < tsp; ++j) {
-            // Prefetch the next column while this one is multiplied.
+            // Prefetch the next column while this one is multiplied (the last
+            // step re-reads its own column).
             v4i vn[KSTEPS];
-            const bool more = (j + 1 < tsp) && t_valid;
+            const int jn = j + 1 < tsp ? j + 1 : j;
 #pragma unroll
-            for (int s = 0; s < KSTEPS; ++s)
-                vn[s] = more ? load_kslice(vcol + (size_t)(j + 1) * K, s * 64 + g * 16, K, al16)
-                             : v4i{0, 0, 0, 0};
+            for (int s = 0; s < KSTEPS; ++s) vn[s] = load_vslice<AL16>(vcol + (size_t)jn * K, s * 64 + g * 16, K);
 #pragma unroll
             for (int r = 0; r < ROWT; ++r) {
                 v4i acc = {0, 0, 0, 0};
@@ -161,10 +177,10 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
                         *reinterpret_cast<v4i*>(dst) = acc;
                     }
                 }
-                const double x0 = (double)acc[0], x1 = (double)acc[1];
-                const double x2 = (double)acc[2], x3 = (double)acc[3];
-                pw[r][0] += x0 * x0 + x1 * x1;
-                pw[r][1] += x2 * x2 + x3 * x3;
+                pw[r][0] += (long long)acc[0] * acc[0];
+                pw[r][0] += (long long)acc[1] * acc[1];
+                pw[r][1] += (long long)acc[2] * acc[2];
+                pw[r][1] += (long long)acc[3] * acc[3];
             }
 #pragma unroll
             for (int s = 0; s < KSTEPS; ++s) v[s] = vn[s];
@@ -178,8 +194,8 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
         for (int r = 0; r < ROWT; ++r) {
             const int b0 = beam_base + r * 8 + 2 * g;
             if (b0 < p.B) {
-                dst[b0] = pw[r][0];
-                dst[b0 + 1] = pw[r][1];
+                dst[b0] = (double)pw[r][0];
+                dst[b0 + 1] = (double)pw[r][1];
             }
         }
         return;
@@ -194,20 +210,469 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
     for (int r = 0; r < ROWT; ++r) {
         const int b0 = beam_base + r * 8 + 2 * g;
         if (b0 < p.B) {
-            out[heap_base + (size_t)b0 * tf_size] = quantise(pw[r][0], off, sc, p.quant);
-            out[heap_base + (size_t)(b0 + 1) * tf_size] = quantise(pw[r][1], off, sc, p.quant);
+            out[heap_base + (size_t)b0 * tf_size] = quantise((double)pw[r][0], off, sc, p.quant);
+            out[heap_base + (size_t)(b0 + 1) * tf_size] = quantise((double)pw[r][1], off, sc, p.quant);
         }
     }
 }
 
-template <int ROWT, int MODE>
+// ---------------------------------------------------------------------------
+// v2: v_mfma_i32_32x32x32_i8 (the full-rate int8 MFMA on gfx950).
+//
+// A (32 rows x 32 k) = embedded weights, lane l: row l%32, K half h = l/32.
+// B (32 k x 32 cols) = voltages, lane l: column l%32, K half h.
+// D lane l: column l%32, rows 8*(i/4) + 4*h + i%4 for i = 0..15, so a lane
+// holds Re and Im of beams 4*q + 2*h + {0, 1} (q = 0..3) of its row tile.
+// K is split into 16-byte slices; slice (h, s) covers bytes
+// [16 * (h * KSTEPS + s), +16) of a column, so each lane reads one contiguous
+// 16*KSTEPS-byte half of every voltage column.  The 32 columns of a tile are
+// 32 output samples; column j of the integration window is processed at
+// step j, with two columns in flight (va / vb) so that loads overlap the
+// MFMAs and the FP64 detection of the previous column.
+// ---------------------------------------------------------------------------
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+template <int KSTEPS, bool AL16>
+__device__ __forceinline__ void cb32_load_col(v4i (&v)[KSTEPS], const int8_t* col, int h, int K)
+{
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) v[s] = load_vslice<AL16>(col, 16 * (h * KSTEPS + s), K);
+}
+
+template <int RT, int KSTEPS, int MODE>
+__device__ __forceinline__ void cb32_column(const v4i (&w)[RT][KSTEPS], const v4i (&v)[KSTEPS],
+                                            long long (&pw)[RT][4][2], const CbArgs& p, int f, long long n,
+                                            int beam_base, int h, bool t_valid)
+{
+    v16i acc[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        acc[r] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s)
+            acc[r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(w[r][s], v[s], acc[r], 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        if (MODE == MODE_SUMS) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int b0 = beam_base + r * 16 + 4 * q + 2 * h;
+                if (t_valid && b0 < p.B) {
+                    int32_t* dst = static_cast<int32_t*>(p.out) +
+                                   (((size_t)f * 2 * p.nsamples + (size_t)n) * p.B + b0) * 2;
+                    *reinterpret_cast<v4i*>(dst) = v4i{acc[r][4 * q], acc[r][4 * q + 1], acc[r][4 * q + 2],
+                                                       acc[r][4 * q + 3]};
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int xr = acc[r][4 * q + 2 * e];
+                const int xi = acc[r][4 * q + 2 * e + 1];
+                pw[r][q][e] += (long long)xr * xr;
+                pw[r][q][e] += (long long)xi * xi;
+            }
+        }
+    }
+}
+
+// MFMA half of cb32_column: acc[r] = W'[r] x column.
+template <int RT, int KSTEPS>
+__device__ __forceinline__ void cb32_mfma(v16i (&acc)[RT], const v4i (&w)[RT][KSTEPS], const v4i (&v)[KSTEPS])
+{
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        acc[r] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s)
+            acc[r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(w[r][s], v[s], acc[r], 0, 0, 0);
+    }
+}
+
+// Detection half of cb32_column: |beam|^2 into the exact int64 accumulators
+// (and, in MODE_SUMS, the raw int32 sums to memory).
+template <int RT, int MODE>
+__device__ __forceinline__ void cb32_detect(const v16i (&acc)[RT], long long (&pw)[RT][4][2], const CbArgs& p,
+                                            int f, long long n, int beam_base, int h, bool t_valid)
+{
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        if (MODE == MODE_SUMS) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int b0 = beam_base + r * 16 + 4 * q + 2 * h;
+                if (t_valid && b0 < p.B) {
+                    int32_t* dst = static_cast<int32_t*>(p.out) +
+                                   (((size_t)f * 2 * p.nsamples + (size_t)n) * p.B + b0) * 2;
+                    *reinterpret_cast<v4i*>(dst) = v4i{acc[r][4 * q], acc[r][4 * q + 1], acc[r][4 * q + 2],
+                                                       acc[r][4 * q + 3]};
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int xr = acc[r][4 * q + 2 * e];
+                const int xi = acc[r][4 * q + 2 * e + 1];
+                pw[r][q][e] += (long long)xr * xr;
+                pw[r][q][e] += (long long)xi * xi;
+            }
+    }
+}
+
+template <int KSTEPS>
+__device__ __forceinline__ void lds_col(v4i (&v)[KSTEPS], const int8_t* src, int jj, int c, int h, int kp)
+{
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s)
+        v[s] = *reinterpret_cast<const v4i*>(src + (jj * 32 + c) * kp + 16 * (h * KSTEPS + s));
+}
+
+template <int RT, int KSTEPS, int MODE, bool AL16>
+__global__ __launch_bounds__(256) void cb_mfma32_k(CbArgs p)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int h = lane >> 5;
+    const int c = lane & 31;
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int bgroup = logical % p.n_bgroups;
+    const int rest = logical / p.n_bgroups;
+    const int tgroup = rest % p.n_tgroups;
+    const int fo = rest / p.n_tgroups;
+    const int wt = 4 / p.wb;
+    const int bc = bgroup * p.wb + (wave % p.wb);
+    const int tc = tgroup * wt + (wave / p.wb);
+    if (bc >= p.n_bchunks || tc >= p.n_tchunks || fo >= p.Fo) return;
+
+    const int K = 2 * p.A;
+    const int beam_base = bc * (16 * RT);
+    const long long t_out = (long long)tc * 32 + c;
+    const bool t_valid = t_out < p.nsamples_out;
+    // Lanes past the last output sample read (and never store) the last one.
+    const long long t_addr = t_valid ? t_out : p.nsamples_out - 1;
+    const int tsp = 2 * p.ts;
+    const long long TP = 2 * p.nsamples;
+
+    long long pw[RT][4][2];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pw[r][q][0] = pw[r][q][1] = 0;
+
+    for (int fc = 0; fc < p.fs; ++fc) {
+        const int f = fo * p.fs + fc;
+        v4i w[RT][KSTEPS];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const int b = beam_base + r * 16 + (c >> 1);
+            const bool bvalid = b < p.B;
+            const int8_t* wcol = p.fba + ((size_t)f * p.B + (bvalid ? b : p.B - 1)) * K;
+#pragma unroll
+            for (int s = 0; s < KSTEPS; ++s) {
+                const v4i raw = load_wslice<AL16>(wcol, 16 * (h * KSTEPS + s), K, bvalid);
+                v4i e;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    e[q] = (int)((c & 1) ? swap_pairs((uint32_t)raw[q]) : neg_odd((uint32_t)raw[q]));
+                w[r][s] = e;
+            }
+        }
+
+        const int8_t* vcol = p.ftpa + ((size_t)f * TP + (size_t)t_addr * tsp) * K;
+        const long long n0 = t_out * tsp;
+        v4i va[KSTEPS], vb[KSTEPS];
+        cb32_load_col<KSTEPS, AL16>(va, vcol, h, K);
+        cb32_load_col<KSTEPS, AL16>(vb, vcol + K, h, K);
+        for (int j = 0; j < tsp; j += 2) {
+            // Two columns in flight; the last steps re-read the final column.
+            const int j2 = j + 2 < tsp ? j + 2 : tsp - 1;
+            const int j3 = j + 3 < tsp ? j + 3 : tsp - 1;
+            v4i cur[KSTEPS];
+#pragma unroll
+            for (int s = 0; s < KSTEPS; ++s) cur[s] = va[s];
+            cb32_load_col<KSTEPS, AL16>(va, vcol + (size_t)j2 * K, h, K);
+            cb32_column<RT, KSTEPS, MODE>(w, cur, pw, p, f, n0 + j, beam_base, h, t_valid);
+#pragma unroll
+            for (int s = 0; s < KSTEPS; ++s) cur[s] = vb[s];
+            cb32_load_col<KSTEPS, AL16>(vb, vcol + (size_t)j3 * K, h, K);
+            cb32_column<RT, KSTEPS, MODE>(w, cur, pw, p, f, n0 + j + 1, beam_base, h, t_valid);
+        }
+    }
+
+    if (MODE == MODE_SUMS || !t_valid) return;
+    if (MODE == MODE_POWER) {
+        double* dst = static_cast<double*>(p.out) + ((size_t)fo * p.nsamples_out + t_out) * p.B;
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int b0 = beam_base + r * 16 + 4 * q + 2 * h;
+                if (b0 < p.B) {
+                    dst[b0] = (double)pw[r][q][0];
+                    dst[b0 + 1] = (double)pw[r][q][1];
+                }
+            }
+        return;
+    }
+    const float off = p.offsets[fo];
+    const float sc = p.scales[fo];
+    int8_t* out = static_cast<int8_t*>(p.out);
+    const size_t tf_size = (size_t)p.ohs * p.Fo;
+    const size_t heap_base = (size_t)(t_out / p.ohs) * p.B * tf_size + (size_t)(t_out % p.ohs) * p.Fo + fo;
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int b0 = beam_base + r * 16 + 4 * q + 2 * h;
+            if (b0 < p.B) {
+                out[heap_base + (size_t)b0 * tf_size] = quantise((double)pw[r][q][0], off, sc, p.quant);
+                out[heap_base + (size_t)(b0 + 1) * tf_size] = quantise((double)pw[r][q][1], off, sc, p.quant);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// v3: 32x32x32 MFMA with the voltage tile staged through LDS.
+//
+// The four waves of a work-group share the same 32 output samples and take
+// 16*RT beams each, so one copy of the voltage columns in LDS feeds all four.
+// The integration window of the 32 samples is streamed in chunks of JC
+// columns: every thread loads its share of the next chunk from HBM/L2 with
+// wide coalesced loads (each sample's JC columns are contiguous) while the
+// current chunk is multiplied out of the other LDS buffer.  LDS rows are
+// padded to 32*KSTEPS + 16 bytes so that the ds_read_b128 of consecutive
+// lanes fall on distinct bank groups.
+// ---------------------------------------------------------------------------
+constexpr int kLdsMaxPieces = 4;   // per-thread prefetch registers (pieces)
+
+template <int RT, int KSTEPS, int MODE, bool AL16>
+__global__ __launch_bounds__(256) void cb_lds_k(CbArgs p, int jc)
+{
+    extern __shared__ __align__(16) int8_t lds[];
+    constexpr int KP = 32 * KSTEPS + 16;
+    constexpr int PB = AL16 ? 16 : 8;   // bytes per fill piece
+    using Piece = typename std::conditional<AL16, v4i, int2>::type;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int h = lane >> 5;
+    const int c = lane & 31;
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int bgroup = logical % p.n_bgroups;
+    const int rest = logical / p.n_bgroups;
+    const int tgroup = rest % p.n_tgroups;
+    const int fo = rest / p.n_tgroups;
+    if (fo >= p.Fo) return;   // uniform over the work-group
+
+    const int K = 2 * p.A;
+    const int beam_base = (bgroup * 4 + wave) * (16 * RT);
+    const bool wave_active = beam_base < p.B;
+    const long long t0 = (long long)tgroup * 32;
+    const long long t_out = t0 + c;
+    const bool t_valid = t_out < p.nsamples_out;
+    const int tsp = 2 * p.ts;
+    const long long TP = 2 * p.nsamples;
+    const int buf_bytes = jc * 32 * KP;
+    const int pieces_per_run = jc * K / PB;   // one output sample's jc columns
+    const int npieces = 32 * pieces_per_run;
+    const int nchunks = tsp / jc;
+
+    long long pw[RT][4][2];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pw[r][q][0] = pw[r][q][1] = 0;
+
+    for (int fc = 0; fc < p.fs; ++fc) {
+        const int f = fo * p.fs + fc;
+        v4i w[RT][KSTEPS];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const int b = beam_base + r * 16 + (c >> 1);
+            const bool bvalid = b < p.B;
+            const int8_t* wcol = p.fba + ((size_t)f * p.B + (bvalid ? b : p.B - 1)) * K;
+#pragma unroll
+            for (int s = 0; s < KSTEPS; ++s) {
+                const v4i raw = load_wslice<AL16>(wcol, 16 * (h * KSTEPS + s), K, bvalid);
+                v4i e;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    e[q] = (int)((c & 1) ? swap_pairs((uint32_t)raw[q]) : neg_odd((uint32_t)raw[q]));
+                w[r][s] = e;
+            }
+        }
+
+        const int8_t* vbase = p.ftpa + (size_t)f * TP * K;
+        Piece pre[kLdsMaxPieces];
+        auto fetch = [&](int chunk) {
+#pragma unroll
+            for (int u = 0; u < kLdsMaxPieces; ++u) {
+                const int i = tid + 256 * u;
+                const int ii = i < npieces ? i : npieces - 1;
+                const int cc = ii / pieces_per_run;
+                const int o = (ii - cc * pieces_per_run) * PB;
+                long long tc = t0 + cc;
+                tc = tc < p.nsamples_out ? tc : p.nsamples_out - 1;
+                pre[u] = *reinterpret_cast<const Piece*>(vbase + ((size_t)tc * tsp + (size_t)chunk * jc) * K + o);
+            }
+        };
+        auto stash = [&](int buf) {
+            int8_t* dst = lds + buf * buf_bytes;
+#pragma unroll
+            for (int u = 0; u < kLdsMaxPieces; ++u) {
+                const int i = tid + 256 * u;
+                if (i < npieces) {
+                    const int cc = i / pieces_per_run;
+                    const int o = (i - cc * pieces_per_run) * PB;
+                    const int jj = o / K;
+                    const int kb = o - jj * K;
+                    *reinterpret_cast<Piece*>(dst + (jj * 32 + cc) * KP + kb) = pre[u];
+                }
+            }
+        };
+
+        fetch(0);
+        stash(0);
+        __syncthreads();
+        for (int k = 0; k < nchunks; ++k) {
+            if (k + 1 < nchunks) fetch(k + 1);
+            if (wave_active) {
+                // One column at a time: with three waves per SIMD the
+                // hardware interleaves MFMA and detection of different waves
+                // (a two-deep software pipeline costs a wave of occupancy and
+                // measured slower).
+                const int8_t* src = lds + (k & 1) * buf_bytes;
+                const long long n0 = t_out * tsp + (long long)k * jc;
+                for (int jj = 0; jj < jc; ++jj) {
+                    v4i v[KSTEPS];
+                    lds_col<KSTEPS>(v, src, jj, c, h, KP);
+                    v16i acc[RT];
+                    cb32_mfma<RT, KSTEPS>(acc, w, v);
+                    cb32_detect<RT, MODE>(acc, pw, p, f, n0 + jj, beam_base, h, t_valid);
+                }
+            }
+            if (k + 1 < nchunks) stash((k + 1) & 1);
+            __syncthreads();
+        }
+    }
+
+    if (MODE == MODE_SUMS || !t_valid || !wave_active) return;
+    if (MODE == MODE_POWER) {
+        double* dst = static_cast<double*>(p.out) + ((size_t)fo * p.nsamples_out + t_out) * p.B;
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int b0 = beam_base + r * 16 + 4 * q + 2 * h;
+                if (b0 < p.B) {
+                    dst[b0] = (double)pw[r][q][0];
+                    dst[b0 + 1] = (double)pw[r][q][1];
+                }
+            }
+        return;
+    }
+    const float off = p.offsets[fo];
+    const float sc = p.scales[fo];
+    int8_t* out = static_cast<int8_t*>(p.out);
+    const size_t tf_size = (size_t)p.ohs * p.Fo;
+    const size_t heap_base = (size_t)(t_out / p.ohs) * p.B * tf_size + (size_t)(t_out % p.ohs) * p.Fo + fo;
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int b0 = beam_base + r * 16 + 4 * q + 2 * h;
+            if (b0 < p.B) {
+                out[heap_base + (size_t)b0 * tf_size] = quantise((double)pw[r][q][0], off, sc, p.quant);
+                out[heap_base + (size_t)(b0 + 1) * tf_size] = quantise((double)pw[r][q][1], off, sc, p.quant);
+            }
+        }
+}
+
+// Chunk length: the largest power of two <= 16 that divides tsp and keeps
+// one chunk within kLdsMaxPieces pieces per thread.
+inline int cb_lds_chunk(int tsp, int K, bool al16)
+{
+    const int pb = al16 ? 16 : 8;
+    int jc = 16;
+    while (jc > 1 && (tsp % jc != 0 || 32 * jc * K / pb > 256 * kLdsMaxPieces)) jc >>= 1;
+    return jc;
+}
+
+inline size_t cb_lds_bytes(int ksteps, int jc) { return (size_t)2 * jc * 32 * (32 * ksteps + 16); }
+
+template <int RT, int MODE, bool AL16>
+hipError_t launch_lds_rt(const CbArgs& a, int ksteps, int jc, dim3 grid, hipStream_t s)
+{
+    const size_t shm = cb_lds_bytes(ksteps, jc);
+    switch (ksteps) {
+    case 1: cb_lds_k<RT, 1, MODE, AL16><<<grid, 256, shm, s>>>(a, jc); break;
+    case 2: cb_lds_k<RT, 2, MODE, AL16><<<grid, 256, shm, s>>>(a, jc); break;
+    case 3: cb_lds_k<RT, 3, MODE, AL16><<<grid, 256, shm, s>>>(a, jc); break;
+    case 4: cb_lds_k<RT, 4, MODE, AL16><<<grid, 256, shm, s>>>(a, jc); break;
+    case 5: cb_lds_k<RT, 5, MODE, AL16><<<grid, 256, shm, s>>>(a, jc); break;
+    case 6: cb_lds_k<RT, 6, MODE, AL16><<<grid, 256, shm, s>>>(a, jc); break;
+    case 7: cb_lds_k<RT, 7, MODE, AL16><<<grid, 256, shm, s>>>(a, jc); break;
+    case 8: cb_lds_k<RT, 8, MODE, AL16><<<grid, 256, shm, s>>>(a, jc); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_lds_mode(const CbArgs& a, int rt, int ksteps, int jc, dim3 grid, hipStream_t s)
+{
+    const bool al16 = (a.A % 8) == 0;
+    switch (rt) {
+    case 1: return al16 ? launch_lds_rt<1, MODE, true>(a, ksteps, jc, grid, s)
+                        : launch_lds_rt<1, MODE, false>(a, ksteps, jc, grid, s);
+    case 2: return al16 ? launch_lds_rt<2, MODE, true>(a, ksteps, jc, grid, s)
+                        : launch_lds_rt<2, MODE, false>(a, ksteps, jc, grid, s);
+    case 4: return al16 ? launch_lds_rt<4, MODE, true>(a, ksteps, jc, grid, s)
+                        : launch_lds_rt<4, MODE, false>(a, ksteps, jc, grid, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int RT, int MODE, bool AL16>
+hipError_t launch32_rt(const CbArgs& a, int ksteps, dim3 grid, hipStream_t s)
+{
+    switch (ksteps) {
+    case 1: cb_mfma32_k<RT, 1, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    case 2: cb_mfma32_k<RT, 2, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    case 3: cb_mfma32_k<RT, 3, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    case 4: cb_mfma32_k<RT, 4, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    case 5: cb_mfma32_k<RT, 5, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    case 6: cb_mfma32_k<RT, 6, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    case 7: cb_mfma32_k<RT, 7, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    case 8: cb_mfma32_k<RT, 8, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch32_mode(const CbArgs& a, int rt, int ksteps, dim3 grid, hipStream_t s)
+{
+    const bool al16 = (a.A % 8) == 0;
+    if (rt == 4)
+        return al16 ? launch32_rt<4, MODE, true>(a, ksteps, grid, s) : launch32_rt<4, MODE, false>(a, ksteps, grid, s);
+    return al16 ? launch32_rt<2, MODE, true>(a, ksteps, grid, s) : launch32_rt<2, MODE, false>(a, ksteps, grid, s);
+}
+
+template <int ROWT, int MODE, bool AL16>
 hipError_t launch_rowt(const CbArgs& a, int ksteps, dim3 grid, hipStream_t s)
 {
     switch (ksteps) {
-    case 1: cb_mfma_k<ROWT, 1, MODE><<<grid, 256, 0, s>>>(a); break;
-    case 2: cb_mfma_k<ROWT, 2, MODE><<<grid, 256, 0, s>>>(a); break;
-    case 3: cb_mfma_k<ROWT, 3, MODE><<<grid, 256, 0, s>>>(a); break;
-    case 4: cb_mfma_k<ROWT, 4, MODE><<<grid, 256, 0, s>>>(a); break;
+    case 1: cb_mfma_k<ROWT, 1, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    case 2: cb_mfma_k<ROWT, 2, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    case 3: cb_mfma_k<ROWT, 3, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
+    case 4: cb_mfma_k<ROWT, 4, MODE, AL16><<<grid, 256, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -216,7 +681,10 @@ hipError_t launch_rowt(const CbArgs& a, int ksteps, dim3 grid, hipStream_t s)
 template <int MODE>
 hipError_t launch_mode(const CbArgs& a, int rowt, int ksteps, dim3 grid, hipStream_t s)
 {
-    return rowt == 8 ? launch_rowt<8, MODE>(a, ksteps, grid, s) : launch_rowt<4, MODE>(a, ksteps, grid, s);
+    const bool al16 = (a.A % 8) == 0;
+    if (rowt == 8)
+        return al16 ? launch_rowt<8, MODE, true>(a, ksteps, grid, s) : launch_rowt<8, MODE, false>(a, ksteps, grid, s);
+    return al16 ? launch_rowt<4, MODE, true>(a, ksteps, grid, s) : launch_rowt<4, MODE, false>(a, ksteps, grid, s);
 }
 
 }  // namespace
@@ -268,7 +736,7 @@ extern "C" int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets,
     }
     st = fbf_cb_plan_set_levels(p, offsets, scales);
     if (st) {
-        hipFree(p->d_levels);
+        (void)hipFree(p->d_levels);
         delete p;
         return st;
     }
@@ -294,13 +762,35 @@ extern "C" int fbf_cb_plan_destroy(fbf_cb_plan* plan)
     return FBF_OK;
 }
 
-// Variant table: rows per wave tile.  0 = automatic.
-static const int kCbVariantRowt[] = {0, 8, 4};
+// Kernel variants: MFMA shape and beams per wave.  Variant 0 picks
+// automatically (the tuned choice for the geometry).
+struct CbVariant {
+    int mfma;        // 32: v_mfma_i32_32x32x32_i8, 16: v_mfma_i32_16x16x64_i8
+    int rows;        // row tiles per wave (RT for 32x32, ROWT for 16x16)
+    bool lds;        // voltage tile staged through LDS and shared by the work-group
+};
+static const CbVariant kCbVariants[] = {
+    {0, 0, false},   // auto
+    {32, 4, false},  // 64 beams x 32 samples per wave
+    {32, 2, false},  // 32 beams x 32 samples per wave
+    {16, 8, false},  // 64 beams x 16 samples per wave
+    {16, 4, false},  // 32 beams x 16 samples per wave
+    {32, 1, true},   // LDS: 16 beams per wave, 64 per work-group
+    {32, 2, true},   // LDS: 32 beams per wave, 128 per work-group
+    {32, 4, true},   // LDS: 64 beams per wave, 256 per work-group
+};
+
+static CbVariant cb_pick_variant(const fbf_cb_desc& d, int variant)
+{
+    if (variant > 0) return kCbVariants[variant];
+    (void)d;
+    return kCbVariants[5];
+}
 
 extern "C" int fbf_cb_variant_count(int* count)
 {
     FBF_REQUIRE(count, FBF_ERR_ARG, "fbf_cb_variant_count: null pointer");
-    *count = (int)(sizeof(kCbVariantRowt) / sizeof(kCbVariantRowt[0]));
+    *count = (int)(sizeof(kCbVariants) / sizeof(kCbVariants[0]));
     return FBF_OK;
 }
 
@@ -367,9 +857,13 @@ static int cb_run(fbf_cb_plan* plan, int mode, const void* ftpa, size_t ftpa_byt
                 FBF_ERR_ARG, "fbf_cb: buffers must be 16-byte aligned");
     if (T == 0) return FBF_OK;
 
-    int rowt = kCbVariantRowt[plan->variant];
-    if (rowt == 0) rowt = d.nbeams >= 64 ? 8 : 4;
-    const int ksteps = (2 * d.nantennas + 63) / 64;
+    const CbVariant var = cb_pick_variant(d, plan->variant);
+    const int kbytes = var.mfma == 32 ? 32 : 64;
+    const int ksteps = (2 * d.nantennas + kbytes - 1) / kbytes;
+    const int beams_per_wave = var.mfma == 32 ? 16 * var.rows : 8 * var.rows;
+    const int t_per_wave = var.mfma == 32 ? 32 : 16;
+    const bool al16 = d.nantennas % 8 == 0;
+    const int jc = cb_lds_chunk(2 * d.tscrunch, 2 * d.nantennas, al16);
     CbArgs a;
     a.ftpa = static_cast<const int8_t*>(ftpa);
     a.fba = static_cast<const int8_t*>(fba);
@@ -386,23 +880,46 @@ static int cb_run(fbf_cb_plan* plan, int mode, const void* ftpa, size_t ftpa_byt
     a.fs = d.fscrunch;
     a.ohs = plan->ohs;
     a.quant = d.quant_mode;
-    a.n_bchunks = (d.nbeams + 8 * rowt - 1) / (8 * rowt);
-    a.n_tchunks = (int)((To + 15) / 16);
-    a.wb = a.n_bchunks >= 4 ? 4 : (a.n_bchunks >= 2 ? 2 : 1);
-    a.n_bgroups = (a.n_bchunks + a.wb - 1) / a.wb;
-    const int wt = 4 / a.wb;
-    a.n_tgroups = (a.n_tchunks + wt - 1) / wt;
+    a.n_bchunks = (d.nbeams + beams_per_wave - 1) / beams_per_wave;
+    a.n_tchunks = (int)((To + t_per_wave - 1) / t_per_wave);
+    if (var.lds) {
+        // Four waves share the samples and split the beams.
+        a.wb = 4;
+        a.n_bgroups = (a.n_bchunks + 3) / 4;
+        a.n_tgroups = a.n_tchunks;
+    } else {
+        a.wb = a.n_bchunks >= 4 ? 4 : (a.n_bchunks >= 2 ? 2 : 1);
+        a.n_bgroups = (a.n_bchunks + a.wb - 1) / a.wb;
+        const int wt = 4 / a.wb;
+        a.n_tgroups = (a.n_tchunks + wt - 1) / wt;
+    }
     const long long nblocks = (long long)Fo * a.n_tgroups * a.n_bgroups;
     FBF_REQUIRE(nblocks > 0 && nblocks < (1ll << 31), FBF_ERR_UNSUPPORTED, "fbf_cb: grid too large");
     hipStream_t s = static_cast<hipStream_t>(stream);
     dim3 grid((unsigned)nblocks);
     hipError_t e;
-    if (mode == MODE_INT8)
-        e = launch_mode<MODE_INT8>(a, rowt, ksteps, grid, s);
-    else if (mode == MODE_SUMS)
-        e = launch_mode<MODE_SUMS>(a, rowt, ksteps, grid, s);
-    else
-        e = launch_mode<MODE_POWER>(a, rowt, ksteps, grid, s);
+    if (var.lds) {
+        if (mode == MODE_INT8)
+            e = launch_lds_mode<MODE_INT8>(a, var.rows, ksteps, jc, grid, s);
+        else if (mode == MODE_SUMS)
+            e = launch_lds_mode<MODE_SUMS>(a, var.rows, ksteps, jc, grid, s);
+        else
+            e = launch_lds_mode<MODE_POWER>(a, var.rows, ksteps, jc, grid, s);
+    } else if (var.mfma == 32) {
+        if (mode == MODE_INT8)
+            e = launch32_mode<MODE_INT8>(a, var.rows, ksteps, grid, s);
+        else if (mode == MODE_SUMS)
+            e = launch32_mode<MODE_SUMS>(a, var.rows, ksteps, grid, s);
+        else
+            e = launch32_mode<MODE_POWER>(a, var.rows, ksteps, grid, s);
+    } else {
+        if (mode == MODE_INT8)
+            e = launch_mode<MODE_INT8>(a, var.rows, ksteps, grid, s);
+        else if (mode == MODE_SUMS)
+            e = launch_mode<MODE_SUMS>(a, var.rows, ksteps, grid, s);
+        else
+            e = launch_mode<MODE_POWER>(a, var.rows, ksteps, grid, s);
+    }
     if (e != hipSuccess) return hip_status(e, "cb_mfma_k launch");
     return FBF_OK;
 }

@@ -147,7 +147,7 @@ extern "C" int fbf_ib_plan_create(const fbf_ib_desc* desc, const float* offsets,
     }
     int st = fbf_ib_plan_set_levels(p, offsets, scales);
     if (st) {
-        hipFree(p->d_levels);
+        (void)hipFree(p->d_levels);
         delete p;
         return st;
     }

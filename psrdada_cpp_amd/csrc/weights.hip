@@ -68,7 +68,7 @@ extern "C" int fbf_weights_plan_create(int nantennas, int nbeams, int nchans, co
     hipError_t e = hipMalloc(&p->d_freqs, sizeof(double) * nchans);
     if (e == hipSuccess) e = hipMemcpy(p->d_freqs, chan_freqs, sizeof(double) * nchans, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        if (p->d_freqs) hipFree(p->d_freqs);
+        if (p->d_freqs) (void)hipFree(p->d_freqs);
         delete p;
         return hip_status(e, "fbf_weights_plan_create");
     }

@@ -286,3 +286,24 @@ def test_weights_fp64_phase_config4(gpu, oracle, epoch):
     diff = np.abs(w.astype(int) - ref.astype(int))
     assert diff.max() <= 1
     assert (diff > 0).mean() < 1e-3
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("F,T,A,B,ts,fs", [(4, 512, 64, 512, 16, 1), (4, 256, 44, 96, 4, 2), (2, 512, 8, 64, 16, 1),
+                                            (2, 256, 40, 48, 64, 1), (2, 256, 128, 32, 8, 1)])
+def test_cb_every_variant_bit_exact(gpu, oracle, variant, F, T, A, B, ts, fs):
+    """Every kernel variant (MFMA shape x tile) gives the oracle's sums and power."""
+    capi = gpu
+    ftpa = gaussian_int8(1300 + A + variant, F * T * 2 * A * 2, 16.0)
+    w = unit_weights(1400 + B, F * B * A)
+    off, sc = _levels(oracle, A, ts, fs, F)
+    plan = capi.CbPlan(A, B, F, ts, fs, off, sc, nsamples_per_heap_out=T // ts)
+    plan.set_variant(variant)
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    d_s = capi.DeviceBuffer(F * T * 2 * B * 2 * 4)
+    plan.beam_sums(d_v, d_w, d_s)
+    np.testing.assert_array_equal(d_s.download(np.int32), oracle.cb_sums(ftpa, w, F, T, A, B).reshape(-1))
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, fs)
+    d_o = capi.DeviceBuffer((F // fs) * (T // ts) * B)
+    plan.beamform(d_v, d_w, d_o)
+    np.testing.assert_array_equal(d_o.download(), oracle.cb_tbtf(p, off, sc, T // ts))

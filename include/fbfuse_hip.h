@@ -143,6 +143,24 @@ int fbf_cb_output_bytes(const fbf_cb_plan* plan, size_t ftpa_bytes, size_t* tbtf
 int fbf_cb_beamform(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
                     const void* fba_weights, size_t weights_bytes,
                     void* tbtf, size_t tbtf_bytes, fbf_stream_t stream);
+/* Beamform the plan's antennas as a subarray of a full-antenna FTPA block:
+ * ftpa_block holds block_nantennas antennas per column and this subarray's
+ * antennas start at antenna_offset.  Same output as splitting the subset
+ * out first (SplitTranspose + fbf_cb_beamform) without the copy.  No
+ * counterpart in the reference; used by the batched multi-subarray path. */
+int fbf_cb_beamform_subarray(fbf_cb_plan* plan, const void* ftpa_block, size_t block_bytes,
+                             int block_nantennas, int antenna_offset,
+                             const void* fba_weights, size_t weights_bytes,
+                             void* tbtf, size_t tbtf_bytes, fbf_stream_t stream);
+/* LDS chunk length (columns of the integration window staged per step),
+ * 0 = automatic; a power of two <= 16 dividing 2*tscrunch. */
+int fbf_cb_plan_set_chunk(fbf_cb_plan* plan, int chunk);
+/* Times every kernel variant and chunk length on synthetic data of the
+ * plan's geometry and nsamples samples, keeps the fastest on the plan and
+ * in a process-wide table (later plans of the same geometry start from it).
+ * Needs scratch device memory for one block.  best_* may be NULL. */
+int fbf_cb_autotune(fbf_cb_plan* plan, long long nsamples, int reps,
+                    int* best_variant, int* best_chunk, float* best_ms);
 /* Exact int32 beam sums, int32 [F][T][P][B][2] (re, im). */
 int fbf_cb_beam_sums(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
                      const void* fba_weights, size_t weights_bytes,
@@ -188,6 +206,45 @@ int fbf_weights_plan_destroy(fbf_weights_plan* plan);
 int fbf_weights_generate(fbf_weights_plan* plan, const void* delays, size_t delays_bytes,
                          double epoch, void* weights, size_t weights_bytes,
                          float* phase, fbf_stream_t stream);
+
+/* ---- batched subarrays on one block (no counterpart in the reference:
+ * it compiles one subarray per process, fbfuse_constants.hpp:12-14) ---- */
+typedef struct fbf_subarray_desc {
+    int start_antenna;          /* first antenna of the contiguous subset */
+    int nantennas;
+    int nbeams;
+    int tscrunch;
+    int fscrunch;
+    int nsamples_per_heap_out;  /* 0 -> 8192 / (F / fscrunch) */
+    int quant_mode;
+} fbf_subarray_desc;
+
+typedef struct fbf_batch fbf_batch;
+
+/* One TAFTP block of total_nantennas x nchans x nheap_groups heaps of
+ * nsamples_per_heap samples, beamformed as nsub subarrays.  offsets[i] and
+ * scales[i] are host arrays of nchans / subs[i].fscrunch floats. */
+int fbf_batch_create(int total_nantennas, int nchans, int nsamples_per_heap, int nheap_groups,
+                     int nsub, const fbf_subarray_desc* subs,
+                     const float* const* offsets, const float* const* scales, fbf_batch** batch);
+int fbf_batch_destroy(fbf_batch* batch);
+/* Adds the incoherent beam over all antennas of the block (its output is
+ * outputs[nsub] in fbf_batch_run). */
+int fbf_batch_enable_ib(fbf_batch* batch, int tscrunch, int fscrunch, int quant_mode,
+                        const float* offsets, const float* scales);
+int fbf_batch_output_bytes(const fbf_batch* batch, int index, size_t* bytes);
+int fbf_batch_weights_bytes(const fbf_batch* batch, int sub, size_t* bytes);
+int fbf_batch_set_variant(fbf_batch* batch, int sub, int variant, int chunk);
+int fbf_batch_autotune(fbf_batch* batch, int reps);
+/* Split-transposes the block once into a full-antenna FTPA workspace and
+ * runs every subarray's coherent beamformer on it (and the incoherent beam
+ * on the TAFTP block), as independent branches forked from `stream` and
+ * joined back into it.  use_graph != 0 captures this once into a hipGraph and
+ * replays it; the graph is captured again whenever a buffer pointer
+ * changes.  weights[i]: device FBA weights of subarray i; outputs: device
+ * outputs (TBTF per subarray, then TF if the incoherent beam is enabled). */
+int fbf_batch_run(fbf_batch* batch, const void* taftp, size_t taftp_bytes,
+                  const void* const* weights, void* const* outputs, int use_graph, fbf_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,12 @@ class CbDesc(C.Structure):
                 ("nsamples_per_heap_out", C.c_int), ("quant_mode", C.c_int)]
 
 
+class SubarrayDesc(C.Structure):
+    _fields_ = [("start_antenna", C.c_int), ("nantennas", C.c_int), ("nbeams", C.c_int),
+                ("tscrunch", C.c_int), ("fscrunch", C.c_int), ("nsamples_per_heap_out", C.c_int),
+                ("quant_mode", C.c_int)]
+
+
 class IbDesc(C.Structure):
     _fields_ = [("nantennas", C.c_int), ("nchans", C.c_int), ("nsamples_per_heap", C.c_int),
                 ("tscrunch", C.c_int), ("fscrunch", C.c_int), ("quant_mode", C.c_int)]
@@ -88,6 +94,18 @@ _SIGS = {
     "fbf_cb_beamform": [vp, vp, sz, vp, sz, vp, sz, vp],
     "fbf_cb_beam_sums": [vp, vp, sz, vp, sz, vp, sz, vp],
     "fbf_cb_beam_power": [vp, vp, sz, vp, sz, vp, sz, vp],
+    "fbf_cb_beamform_subarray": [vp, vp, sz, i32, i32, vp, sz, vp, sz, vp],
+    "fbf_cb_plan_set_chunk": [vp, i32],
+    "fbf_cb_autotune": [vp, C.c_longlong, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)],
+    "fbf_batch_create": [i32, i32, i32, i32, i32, C.POINTER(SubarrayDesc), C.POINTER(C.c_void_p),
+                         C.POINTER(C.c_void_p), pvp],
+    "fbf_batch_destroy": [vp],
+    "fbf_batch_enable_ib": [vp, i32, i32, i32, vp, vp],
+    "fbf_batch_output_bytes": [vp, i32, C.POINTER(C.c_size_t)],
+    "fbf_batch_weights_bytes": [vp, i32, C.POINTER(C.c_size_t)],
+    "fbf_batch_set_variant": [vp, i32, i32, i32],
+    "fbf_batch_autotune": [vp, i32],
+    "fbf_batch_run": [vp, vp, sz, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), i32, vp],
     "fbf_ib_plan_create": [C.POINTER(IbDesc), vp, vp, pvp],
     "fbf_ib_plan_set_levels": [vp, vp, vp],
     "fbf_ib_plan_destroy": [vp],
@@ -324,6 +342,18 @@ class CbPlan:
         call("fbf_cb_beamform", self.handle, ftpa.ptr, ftpa.nbytes, weights.ptr, weights.nbytes,
              out.ptr, out.nbytes, _s(stream))
 
+    def set_chunk(self, c):
+        call("fbf_cb_plan_set_chunk", self.handle, int(c))
+
+    def autotune(self, nsamples, reps=5):
+        v, c, ms = C.c_int(), C.c_int(), C.c_float()
+        call("fbf_cb_autotune", self.handle, int(nsamples), int(reps), C.byref(v), C.byref(c), C.byref(ms))
+        return v.value, c.value, ms.value
+
+    def beamform_subarray(self, ftpa_block, block_nantennas, antenna_offset, weights, out, stream=None):
+        call("fbf_cb_beamform_subarray", self.handle, ftpa_block.ptr, ftpa_block.nbytes, int(block_nantennas),
+             int(antenna_offset), weights.ptr, weights.nbytes, out.ptr, out.nbytes, _s(stream))
+
     def beam_sums(self, ftpa, weights, out, stream=None):
         call("fbf_cb_beam_sums", self.handle, ftpa.ptr, ftpa.nbytes, weights.ptr, weights.nbytes,
              out.ptr, out.nbytes, _s(stream))
@@ -396,6 +426,67 @@ class WeightsPlan:
     def close(self):
         if self.handle:
             call("fbf_weights_plan_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched subarrays on one block (fbf_batch)."""
+
+    def __init__(self, total_nantennas, nchans, nheap_groups, subarrays, levels, nsamples_per_heap=256):
+        """subarrays: list of dicts (start_antenna, nantennas, nbeams, tscrunch, fscrunch,
+        [nsamples_per_heap_out], [quant_mode]); levels: list of (offsets, scales)."""
+        n = len(subarrays)
+        arr = (SubarrayDesc * n)()
+        for i, sd in enumerate(subarrays):
+            arr[i] = SubarrayDesc(sd["start_antenna"], sd["nantennas"], sd["nbeams"], sd["tscrunch"],
+                                  sd["fscrunch"], sd.get("nsamples_per_heap_out", 0), sd.get("quant_mode", 0))
+        self._levels = [(np.ascontiguousarray(o, np.float32), np.ascontiguousarray(s, np.float32))
+                        for o, s in levels]
+        offs = (C.c_void_p * n)(*[_np_ptr(o).value for o, _ in self._levels])
+        scs = (C.c_void_p * n)(*[_np_ptr(s).value for _, s in self._levels])
+        h = C.c_void_p()
+        call("fbf_batch_create", total_nantennas, nchans, nsamples_per_heap, nheap_groups, n, arr, offs, scs,
+             C.byref(h))
+        self.handle = h
+        self.n = n
+        self.ib = False
+
+    def enable_ib(self, tscrunch, fscrunch, offsets, scales, quant_mode=QUANT_ROUND):
+        o = np.ascontiguousarray(offsets, np.float32)
+        s = np.ascontiguousarray(scales, np.float32)
+        call("fbf_batch_enable_ib", self.handle, tscrunch, fscrunch, quant_mode, _np_ptr(o), _np_ptr(s))
+        self.ib = True
+
+    def output_bytes(self, i):
+        b = C.c_size_t()
+        call("fbf_batch_output_bytes", self.handle, i, C.byref(b))
+        return b.value
+
+    def weights_bytes(self, i):
+        b = C.c_size_t()
+        call("fbf_batch_weights_bytes", self.handle, i, C.byref(b))
+        return b.value
+
+    def set_variant(self, i, variant, chunk=0):
+        call("fbf_batch_set_variant", self.handle, i, variant, chunk)
+
+    def autotune(self, reps=3):
+        call("fbf_batch_autotune", self.handle, reps)
+
+    def run(self, taftp, weights, outputs, use_graph=True, stream=None):
+        w = (C.c_void_p * self.n)(*[x.ptr.value for x in weights])
+        o = (C.c_void_p * len(outputs))(*[x.ptr.value for x in outputs])
+        call("fbf_batch_run", self.handle, taftp.ptr, taftp.nbytes, w, o, int(bool(use_graph)), _s(stream))
+
+    def close(self):
+        if self.handle:
+            call("fbf_batch_destroy", self.handle)
             self.handle = None
 
     def __del__(self):

@@ -27,7 +27,11 @@
 // A <= 128) and handed to the quantiser as a double, like the oracle.
 #include "fbf_internal.hpp"
 
+#include <map>
+#include <mutex>
+#include <string>
 #include <type_traits>
+#include <vector>
 
 namespace fbf {
 namespace {
@@ -46,6 +50,11 @@ struct CbArgs {
     long long nsamples_out;
     int A, B, F, Fo, ts, fs, ohs, quant;
     int n_bchunks, n_tchunks, n_bgroups, n_tgroups, wb;
+    // Voltage columns are `vstride` bytes apart and this beamformer's
+    // 2*A bytes start `voff` bytes into each column (vstride = 2*A and
+    // voff = 0 for a split-transposed subset; a subarray reading a full
+    // FTPA block uses the block's column stride and its antenna offset).
+    int vstride, voff;
 };
 
 // Loads of one 16-byte K slice [k0, k0 + 16) of a column of K bytes.  They
@@ -150,7 +159,7 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
             }
         }
 
-        const int8_t* vcol = p.ftpa + ((size_t)f * TP + (size_t)t_addr * tsp) * K;
+        const int8_t* vcol = p.ftpa + ((size_t)f * TP + (size_t)t_addr * tsp) * p.vstride + p.voff;
         v4i v[KSTEPS];
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) v[s] = load_vslice<AL16>(vcol, s * 64 + g * 16, K);
@@ -161,7 +170,7 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
             v4i vn[KSTEPS];
             const int jn = j + 1 < tsp ? j + 1 : j;
 #pragma unroll
-            for (int s = 0; s < KSTEPS; ++s) vn[s] = load_vslice<AL16>(vcol + (size_t)jn * K, s * 64 + g * 16, K);
+            for (int s = 0; s < KSTEPS; ++s) vn[s] = load_vslice<AL16>(vcol + (size_t)jn * p.vstride, s * 64 + g * 16, K);
 #pragma unroll
             for (int r = 0; r < ROWT; ++r) {
                 v4i acc = {0, 0, 0, 0};
@@ -383,11 +392,11 @@ __global__ __launch_bounds__(256) void cb_mfma32_k(CbArgs p)
             }
         }
 
-        const int8_t* vcol = p.ftpa + ((size_t)f * TP + (size_t)t_addr * tsp) * K;
+        const int8_t* vcol = p.ftpa + ((size_t)f * TP + (size_t)t_addr * tsp) * p.vstride + p.voff;
         const long long n0 = t_out * tsp;
         v4i va[KSTEPS], vb[KSTEPS];
         cb32_load_col<KSTEPS, AL16>(va, vcol, h, K);
-        cb32_load_col<KSTEPS, AL16>(vb, vcol + K, h, K);
+        cb32_load_col<KSTEPS, AL16>(vb, vcol + p.vstride, h, K);
         for (int j = 0; j < tsp; j += 2) {
             // Two columns in flight; the last steps re-read the final column.
             const int j2 = j + 2 < tsp ? j + 2 : tsp - 1;
@@ -395,11 +404,11 @@ __global__ __launch_bounds__(256) void cb_mfma32_k(CbArgs p)
             v4i cur[KSTEPS];
 #pragma unroll
             for (int s = 0; s < KSTEPS; ++s) cur[s] = va[s];
-            cb32_load_col<KSTEPS, AL16>(va, vcol + (size_t)j2 * K, h, K);
+            cb32_load_col<KSTEPS, AL16>(va, vcol + (size_t)j2 * p.vstride, h, K);
             cb32_column<RT, KSTEPS, MODE>(w, cur, pw, p, f, n0 + j, beam_base, h, t_valid);
 #pragma unroll
             for (int s = 0; s < KSTEPS; ++s) cur[s] = vb[s];
-            cb32_load_col<KSTEPS, AL16>(vb, vcol + (size_t)j3 * K, h, K);
+            cb32_load_col<KSTEPS, AL16>(vb, vcol + (size_t)j3 * p.vstride, h, K);
             cb32_column<RT, KSTEPS, MODE>(w, cur, pw, p, f, n0 + j + 1, beam_base, h, t_valid);
         }
     }
@@ -508,7 +517,7 @@ __global__ __launch_bounds__(256) void cb_lds_k(CbArgs p, int jc)
             }
         }
 
-        const int8_t* vbase = p.ftpa + (size_t)f * TP * K;
+        const int8_t* vbase = p.ftpa + (size_t)f * TP * p.vstride + p.voff;
         Piece pre[kLdsMaxPieces];
         auto fetch = [&](int chunk) {
 #pragma unroll
@@ -519,7 +528,9 @@ __global__ __launch_bounds__(256) void cb_lds_k(CbArgs p, int jc)
                 const int o = (ii - cc * pieces_per_run) * PB;
                 long long tc = t0 + cc;
                 tc = tc < p.nsamples_out ? tc : p.nsamples_out - 1;
-                pre[u] = *reinterpret_cast<const Piece*>(vbase + ((size_t)tc * tsp + (size_t)chunk * jc) * K + o);
+                const int jj = o / K;
+                pre[u] = *reinterpret_cast<const Piece*>(vbase + ((size_t)tc * tsp + (size_t)chunk * jc + jj) * p.vstride +
+                                                         (o - jj * K));
             }
         };
         auto stash = [&](int buf) {
@@ -699,6 +710,7 @@ struct fbf_cb_plan {
     fbf_cb_desc d;
     int ohs;
     int variant;
+    int chunk;        // LDS chunk length in columns, 0 = automatic
     float* d_levels;  // device: offsets[Fo] then scales[Fo]
 };
 
@@ -717,6 +729,18 @@ static int cb_validate(const fbf_cb_desc* d)
     return FBF_OK;
 }
 
+// Process-wide autotuning table: geometry -> (variant, chunk).
+namespace {
+std::mutex g_tune_mutex;
+std::map<std::string, std::pair<int, int>> g_tune_table;
+
+std::string tune_key(const fbf_cb_desc& d)
+{
+    return std::to_string(d.nantennas) + "/" + std::to_string(d.nbeams) + "/" + std::to_string(d.nchans) + "/" +
+           std::to_string(d.tscrunch) + "/" + std::to_string(d.fscrunch);
+}
+}  // namespace
+
 extern "C" int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets, const float* scales,
                                   fbf_cb_plan** plan)
 {
@@ -728,7 +752,7 @@ extern "C" int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets,
     const int Fo = desc->nchans / desc->fscrunch;
     int ohs = desc->nsamples_per_heap_out ? desc->nsamples_per_heap_out : 8192 / Fo;
     FBF_REQUIRE(ohs > 0, FBF_ERR_ARG, "fbf_cb_plan_create: output heap holds no samples");
-    auto* p = new fbf_cb_plan{*desc, ohs, 0, nullptr};
+    auto* p = new fbf_cb_plan{*desc, ohs, 0, 0, nullptr};
     hipError_t e = hipMalloc(&p->d_levels, sizeof(float) * 2 * Fo);
     if (e != hipSuccess) {
         delete p;
@@ -739,6 +763,14 @@ extern "C" int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets,
         (void)hipFree(p->d_levels);
         delete p;
         return st;
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tune_table.find(tune_key(*desc));
+        if (it != g_tune_table.end()) {
+            p->variant = it->second.first;
+            p->chunk = it->second.second;
+        }
     }
     *plan = p;
     return FBF_OK;
@@ -824,15 +856,19 @@ extern "C" int fbf_cb_output_bytes(const fbf_cb_plan* plan, size_t ftpa_bytes, s
     return FBF_OK;
 }
 
-static int cb_run(fbf_cb_plan* plan, int mode, const void* ftpa, size_t ftpa_bytes, const void* fba,
-                  size_t weights_bytes, void* out, size_t out_bytes, fbf_stream_t stream)
+// Validates and launches one coherent-beam kernel over T samples of voltage
+// columns that are `vstride` bytes apart, the beamformer's antennas starting
+// `voff` bytes into each column.
+static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T, int vstride, int voff,
+                     const void* fba, size_t weights_bytes, void* out, size_t out_bytes, fbf_stream_t stream)
 {
     FBF_REQUIRE(plan, FBF_ERR_ARG, "fbf_cb: null plan");
     FBF_REQUIRE(ftpa && fba && out, FBF_ERR_ARG, "fbf_cb: null buffer");
     const fbf_cb_desc& d = plan->d;
-    long long T = 0;
-    int st = cb_nsamples(plan, ftpa_bytes, &T);
-    if (st) return st;
+    FBF_REQUIRE(T % d.tscrunch == 0, FBF_ERR_SIZE, "fbf_cb: nsamples is not a multiple of tscrunch");
+    FBF_REQUIRE(vstride >= 2 * d.nantennas + voff && voff >= 0 && vstride % 8 == 0 && voff % 8 == 0,
+                FBF_ERR_ARG, "fbf_cb: bad voltage column stride/offset");
+    const bool al16_view = (vstride % 16 == 0) && (voff % 16 == 0);
     const size_t wbytes = (size_t)2 * d.nchans * d.nbeams * d.nantennas;
     FBF_REQUIRE(weights_bytes == wbytes, FBF_ERR_SIZE,
                 "fbf_cb: weights size " + std::to_string(weights_bytes) + " != expected " +
@@ -863,7 +899,14 @@ static int cb_run(fbf_cb_plan* plan, int mode, const void* ftpa, size_t ftpa_byt
     const int beams_per_wave = var.mfma == 32 ? 16 * var.rows : 8 * var.rows;
     const int t_per_wave = var.mfma == 32 ? 32 : 16;
     const bool al16 = d.nantennas % 8 == 0;
-    const int jc = cb_lds_chunk(2 * d.tscrunch, 2 * d.nantennas, al16);
+    FBF_REQUIRE(!al16 || al16_view, FBF_ERR_ARG,
+                "fbf_cb: a multiple-of-8 antenna subset needs a 16-byte aligned column stride and offset");
+    int jc = cb_lds_chunk(2 * d.tscrunch, 2 * d.nantennas, al16);
+    if (plan->chunk > 0) {
+        FBF_REQUIRE((2 * d.tscrunch) % plan->chunk == 0 && plan->chunk <= jc, FBF_ERR_ARG,
+                    "fbf_cb: chunk must divide 2*tscrunch and fit the prefetch registers");
+        jc = plan->chunk;
+    }
     CbArgs a;
     a.ftpa = static_cast<const int8_t*>(ftpa);
     a.fba = static_cast<const int8_t*>(fba);
@@ -880,6 +923,8 @@ static int cb_run(fbf_cb_plan* plan, int mode, const void* ftpa, size_t ftpa_byt
     a.fs = d.fscrunch;
     a.ohs = plan->ohs;
     a.quant = d.quant_mode;
+    a.vstride = vstride;
+    a.voff = voff;
     a.n_bchunks = (d.nbeams + beams_per_wave - 1) / beams_per_wave;
     a.n_tchunks = (int)((To + t_per_wave - 1) / t_per_wave);
     if (var.lds) {
@@ -924,6 +969,40 @@ static int cb_run(fbf_cb_plan* plan, int mode, const void* ftpa, size_t ftpa_byt
     return FBF_OK;
 }
 
+static int cb_run(fbf_cb_plan* plan, int mode, const void* ftpa, size_t ftpa_bytes, const void* fba,
+                  size_t weights_bytes, void* out, size_t out_bytes, fbf_stream_t stream)
+{
+    FBF_REQUIRE(plan, FBF_ERR_ARG, "fbf_cb: null plan");
+    long long T = 0;
+    int st = cb_nsamples(plan, ftpa_bytes, &T);
+    if (st) return st;
+    return cb_launch(plan, mode, ftpa, T, 2 * plan->d.nantennas, 0, fba, weights_bytes, out, out_bytes, stream);
+}
+
+extern "C" int fbf_cb_beamform_subarray(fbf_cb_plan* plan, const void* ftpa_block, size_t block_bytes,
+                                        int block_nantennas, int antenna_offset, const void* fba_weights,
+                                        size_t weights_bytes, void* tbtf, size_t tbtf_bytes, fbf_stream_t stream)
+{
+    FBF_REQUIRE(plan, FBF_ERR_ARG, "fbf_cb_beamform_subarray: null plan");
+    FBF_REQUIRE(block_nantennas > 0 && antenna_offset >= 0 &&
+                    antenna_offset + plan->d.nantennas <= block_nantennas,
+                FBF_ERR_ARG, "fbf_cb_beamform_subarray: subarray outside the block's antennas");
+    const size_t per_sample = (size_t)2 * 2 * block_nantennas * plan->d.nchans;
+    FBF_REQUIRE(block_bytes % per_sample == 0, FBF_ERR_SIZE,
+                "fbf_cb_beamform_subarray: block is not a whole number of samples");
+    const long long T = (long long)(block_bytes / per_sample);
+    return cb_launch(plan, MODE_INT8, ftpa_block, T, 2 * block_nantennas, 2 * antenna_offset, fba_weights,
+                     weights_bytes, tbtf, tbtf_bytes, stream);
+}
+
+extern "C" int fbf_cb_plan_set_chunk(fbf_cb_plan* plan, int chunk)
+{
+    FBF_REQUIRE(plan && chunk >= 0 && chunk <= 16 && (chunk & (chunk - 1)) == 0, FBF_ERR_ARG,
+                "fbf_cb_plan_set_chunk: chunk must be 0 (auto) or a power of two <= 16");
+    plan->chunk = chunk;
+    return FBF_OK;
+}
+
 extern "C" int fbf_cb_beamform(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes, const void* fba_weights,
                                size_t weights_bytes, void* tbtf, size_t tbtf_bytes, fbf_stream_t stream)
 {
@@ -940,4 +1019,91 @@ extern "C" int fbf_cb_beam_power(fbf_cb_plan* plan, const void* ftpa, size_t ftp
                                  size_t weights_bytes, double* power, size_t power_bytes, fbf_stream_t stream)
 {
     return cb_run(plan, MODE_POWER, ftpa, ftpa_bytes, fba_weights, weights_bytes, power, power_bytes, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: times every (variant, chunk) candidate on synthetic data of the
+// plan's geometry.  The voltage values do not change the instruction stream,
+// so constant bytes are enough for timing.
+// ---------------------------------------------------------------------------
+extern "C" int fbf_cb_autotune(fbf_cb_plan* plan, long long nsamples, int reps, int* best_variant, int* best_chunk,
+                               float* best_ms)
+{
+    FBF_REQUIRE(plan && nsamples > 0 && reps > 0, FBF_ERR_ARG, "fbf_cb_autotune: bad argument");
+    const fbf_cb_desc& d = plan->d;
+    FBF_REQUIRE(nsamples % d.tscrunch == 0 && (nsamples / d.tscrunch) % plan->ohs == 0, FBF_ERR_ARG,
+                "fbf_cb_autotune: nsamples must fill whole output heaps");
+    const size_t vbytes = (size_t)nsamples * 2 * 2 * d.nantennas * d.nchans;
+    const size_t wbytes = (size_t)2 * d.nchans * d.nbeams * d.nantennas;
+    const size_t obytes = (size_t)(nsamples / d.tscrunch) * d.nbeams * (d.nchans / d.fscrunch);
+    void *v = nullptr, *w = nullptr, *o = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (v) (void)hipFree(v);
+        if (w) (void)hipFree(w);
+        if (o) (void)hipFree(o);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+    };
+    hipError_t e = hipMalloc(&v, vbytes);
+    if (e == hipSuccess) e = hipMalloc(&w, wbytes);
+    if (e == hipSuccess) e = hipMalloc(&o, obytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(v, 0x11, vbytes, s);
+    if (e == hipSuccess) e = hipMemsetAsync(w, 0x5A, wbytes, s);
+    if (e != hipSuccess) {
+        cleanup();
+        return hip_status(e, "fbf_cb_autotune setup");
+    }
+    const int nvar = (int)(sizeof(kCbVariants) / sizeof(kCbVariants[0]));
+    const bool al16 = d.nantennas % 8 == 0;
+    const int jc_max = cb_lds_chunk(2 * d.tscrunch, 2 * d.nantennas, al16);
+    const int saved_variant = plan->variant, saved_chunk = plan->chunk;
+    int bv = saved_variant, bc = saved_chunk;
+    float bms = 1e30f;
+    for (int var = 1; var < nvar; ++var) {
+        std::vector<int> chunks{0};
+        if (kCbVariants[var].lds)
+            for (int jc = jc_max / 2; jc >= 1; jc >>= 1) chunks.push_back(jc);
+        for (int jc : chunks) {
+            plan->variant = var;
+            plan->chunk = jc;
+            int st = cb_launch(plan, MODE_INT8, v, nsamples, 2 * d.nantennas, 0, w, wbytes, o, obytes, s);
+            if (st != FBF_OK) continue;   // candidate not applicable to this geometry
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < reps; ++r)
+                cb_launch(plan, MODE_INT8, v, nsamples, 2 * d.nantennas, 0, w, wbytes, o, obytes, s);
+            (void)hipEventRecord(e1, s);
+            e = hipEventSynchronize(e1);
+            if (e != hipSuccess) {
+                plan->variant = saved_variant;
+                plan->chunk = saved_chunk;
+                cleanup();
+                return hip_status(e, "fbf_cb_autotune timing");
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= (float)reps;
+            if (ms < bms) {
+                bms = ms;
+                bv = var;
+                bc = jc;
+            }
+        }
+    }
+    plan->variant = bv;
+    plan->chunk = bc;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tune_table[tune_key(d)] = {bv, bc};
+    }
+    cleanup();
+    if (best_variant) *best_variant = bv;
+    if (best_chunk) *best_chunk = bc;
+    if (best_ms) *best_ms = bms;
+    return FBF_OK;
 }

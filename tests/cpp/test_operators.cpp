@@ -15,7 +15,14 @@
 #include <vector>
 
 #include "fbf_oracle.h"
+#include <thread>
+
+#include "psrdada_cpp/dada_input_stream.hpp"
+#include "psrdada_cpp/dada_write_client.hpp"
 #include "psrdada_cpp/double_device_buffer.hpp"
+#include "psrdada_cpp/meerkat/fbfuse/DelayManager.hpp"
+#include "psrdada_cpp/meerkat/fbfuse/Header.hpp"
+#include "psrdada_cpp/meerkat/fbfuse/Pipeline.hpp"
 #include "psrdada_cpp/meerkat/fbfuse/CoherentBeamformer.hpp"
 #include "psrdada_cpp/meerkat/fbfuse/IncoherentBeamformer.hpp"
 #include "psrdada_cpp/meerkat/fbfuse/PipelineConfig.hpp"
@@ -245,6 +252,296 @@ void error_paths_throw()
     EXPECT(threw);
 }
 
+// ---------------------------------------------------------------------------
+void header_get_set()
+{
+    std::vector<char> buf(4096, 0);
+    RawBytes rb(buf.data(), buf.size(), buf.size());
+    Header h(rb);
+    h.set<long double>("SAMPLE_CLOCK", 856000000.0L);
+    h.set<long double>("SYNC_TIME", 1234.5L);
+    h.set<std::size_t>("SAMPLE_CLOCK_START", 42);
+    EXPECT(h.get<long double>("SAMPLE_CLOCK") == 856000000.0L);
+    EXPECT(h.get<long double>("SYNC_TIME") == 1234.5L);
+    EXPECT(h.get<std::size_t>("SAMPLE_CLOCK_START") == 42);
+    h.set<std::size_t>("SAMPLE_CLOCK_START", 7);   // replace, not append
+    EXPECT(h.get<std::size_t>("SAMPLE_CLOCK_START") == 7);
+    EXPECT(std::string(buf.data()).find("SAMPLE_CLOCK_START 42") == std::string::npos);
+    bool threw = false;
+    try {
+        h.get<long double>("NOT_A_KEY");
+    } catch (std::runtime_error const&) {
+        threw = true;
+    }
+    EXPECT(threw);
+    h.set_string("UNSET_KEY", "unset");
+    threw = false;
+    try {
+        h.get<long double>("UNSET_KEY");
+    } catch (std::runtime_error const&) {
+        threw = true;
+    }
+    EXPECT(threw);
+    h.purge();
+    EXPECT(buf[0] == 0);
+}
+
+void dada_ring_write_read_eod()
+{
+    DadaDB db(4, 64, 2, 256);
+    db.create();
+    MultiLog log("test");
+    DadaWriteClient writer(db.key(), log);
+    std::thread producer([&] {
+        auto& hb = writer.header_stream().next();
+        std::strcpy(hb.ptr(), "HDR 1\n");
+        hb.used_bytes(7);
+        writer.header_stream().release();
+        for (int i = 0; i < 10; ++i) {
+            auto& b = writer.data_stream().next();
+            std::memset(b.ptr(), i, 64);
+            b.used_bytes(64 - i);
+            writer.data_stream().release(i == 9);
+        }
+    });
+    DadaReadClient reader(db.key(), log);
+    auto& h = reader.header_stream().next();
+    EXPECT(std::string(h.ptr()) == "HDR 1\n");
+    reader.header_stream().release();
+    int n = 0;
+    while (true) {
+        auto& b = reader.data_stream().next();
+        EXPECT(b.used_bytes() == (std::size_t)(64 - n) && b.ptr()[0] == (char)n);
+        reader.data_stream().release();
+        ++n;
+        if (reader.data_stream().at_end()) break;
+    }
+    producer.join();
+    EXPECT(n == 10);
+    bool threw = false;
+    try {
+        DadaWriteClient second(db.key(), log);   // one writer per ring
+    } catch (std::runtime_error const&) {
+        threw = true;
+    }
+    EXPECT(threw);
+}
+
+// DelayManagerTester.cu:47-67
+void delay_manager_updates()
+{
+    PipelineConfig config;
+    config.delay_buffer_shm("fbfuse_test_delays_shm");
+    config.delay_buffer_mutex("fbfuse_test_delays_mutex");
+    config.delay_buffer_sem("fbfuse_test_delays_count");
+    DelayEngineSimulator simulator(config);
+    StreamType stream = nullptr;
+    FBF_ERROR_CHECK(fbf_stream_create(&stream));
+    DelayManager delay_manager(config, stream);
+    const std::size_t n = config.cb_nbeams() * config.cb_nantennas();
+    std::memset(simulator.delays(), 0x01, n * sizeof(float2));
+    simulator.header()->epoch = 10.0;
+    simulator.header()->duration = 5.0;
+    simulator.update_delays();
+    auto const& d = delay_manager.delays();
+    std::vector<float2> got = d.to_host();
+    EXPECT(std::memcmp(got.data(), simulator.delays(), n * sizeof(float2)) == 0);
+    EXPECT(delay_manager.epoch() == 10.0 && delay_manager.duration() == 5.0 && delay_manager.update_count() == 1);
+    delay_manager.delays();   // no new update: no copy
+    EXPECT(delay_manager.update_count() == 1);
+    std::memset(simulator.delays(), 0x02, n * sizeof(float2));
+    simulator.update_delays();
+    got = delay_manager.delays().to_host();
+    EXPECT(std::memcmp(got.data(), simulator.delays(), n * sizeof(float2)) == 0 && delay_manager.update_count() == 2);
+    fbf_stream_destroy(stream);
+}
+
+void delay_manager_bad_keys()
+{
+    PipelineConfig config;
+    config.delay_buffer_shm("fbfuse_test_no_such_shm");
+    bool threw = false;
+    try {
+        DelayManager delay_manager(config, nullptr);
+    } catch (std::runtime_error const&) {
+        threw = true;
+    }
+    EXPECT(threw);
+}
+
+// PipelineTester.cu:46-125: rings, null-sink consumers on their own threads,
+// a synthetic header, blocks pushed through, stop mid-stream.
+void pipeline_simple_run()
+{
+    PipelineConfig config;   // reference geometry
+    config.delay_buffer_shm("fbfuse_test_pipe_shm");
+    config.delay_buffer_mutex("fbfuse_test_pipe_mutex");
+    config.delay_buffer_sem("fbfuse_test_pipe_count");
+    config.centre_frequency(1.4e9f);
+    config.bandwidth(56e6f);
+    DelayEngineSimulator simulator(config);
+    const int ntimestamps_per_block = 64;
+    const std::size_t taftp_block_bytes = (std::size_t)ntimestamps_per_block * config.total_nantennas() *
+                                          config.nchans() * config.nsamples_per_heap() * config.npol() * sizeof(char2);
+    const std::size_t cb_block = config.cb_nbeams() * (config.nsamples_per_heap() * ntimestamps_per_block /
+                                                        config.cb_tscrunch()) *
+                                 (config.nchans() / config.cb_fscrunch());
+    DadaDB cb_buffer(8, cb_block, 4, 4096);
+    cb_buffer.create();
+    config.cb_dada_key(cb_buffer.key());
+    const std::size_t ib_block = (config.nsamples_per_heap() * ntimestamps_per_block / config.ib_tscrunch()) *
+                                 (config.nchans() / config.ib_fscrunch());
+    DadaDB ib_buffer(8, ib_block, 4, 4096);
+    ib_buffer.create();
+    config.ib_dada_key(ib_buffer.key());
+    MultiLog log("PipelineTester");
+    DadaWriteClient cb_write_client(config.cb_dada_key(), log);
+    DadaWriteClient ib_write_client(config.ib_dada_key(), log);
+    std::size_t seen_cb = 0;
+    {
+        Pipeline pipeline(config, cb_write_client, ib_write_client, taftp_block_bytes);
+        NullSink null_sink;
+        DadaInputStream<NullSink> cb_consumer(config.cb_dada_key(), log, null_sink);
+        DadaInputStream<NullSink> ib_consumer(config.ib_dada_key(), log, null_sink);
+        std::thread cb_thread([&] { cb_consumer.start(); });
+        std::thread ib_thread([&] { ib_consumer.start(); });
+        std::vector<char> header_buffer(4096, 0);
+        RawBytes header_rb(header_buffer.data(), 4096, 4096);
+        Header header(header_rb);
+        header.set<long double>("SAMPLE_CLOCK", 856000000.0L);
+        header.set<long double>("SYNC_TIME", 0.0L);
+        header.set<std::size_t>("SAMPLE_CLOCK_START", 0);
+        void* input = nullptr;
+        FBF_ERROR_CHECK(fbf_host_malloc(&input, taftp_block_bytes));
+        std::memset(input, 0, taftp_block_bytes);
+        RawBytes input_rb(static_cast<char*>(input), taftp_block_bytes, taftp_block_bytes);
+        pipeline.init(header_rb);
+        for (int ii = 0; ii < 20; ++ii) {
+            if (ii == 10) simulator.update_delays();   // delay model refresh mid-stream
+            pipeline(input_rb);
+        }
+        cb_consumer.stop();
+        ib_consumer.stop();
+        pipeline(input_rb);   // one more block wakes the consumers
+        pipeline(input_rb);
+        cb_thread.join();
+        ib_thread.join();
+        seen_cb = pipeline.call_count();
+        // timestamp of the block processed by call n: (n - 2) * tick / clock
+        const long double tick = 2.0L * config.total_nchans() * config.nsamples_per_heap() * ntimestamps_per_block;
+        EXPECT(std::fabs((double)(pipeline.unix_timestamp() - (seen_cb - 2) * tick / 856000000.0L)) < 1e-9);
+        FBF_ERROR_CHECK(fbf_host_free(input));
+    }
+    EXPECT(seen_cb == 22);
+}
+
+// The beams leaving the pipeline are those of the operators for the same
+// block (block n in, its beams out on call n + 2), with the delay model read
+// from shared memory and weights at each block's unix time.
+struct CaptureSink {
+    std::vector<std::vector<int8_t>> blocks;
+    std::string header;
+    void init(RawBytes& h) { header.assign(h.ptr(), strnlen(h.ptr(), h.total_bytes())); }
+    bool operator()(RawBytes& b)
+    {
+        blocks.emplace_back(b.ptr(), b.ptr() + b.used_bytes());
+        return false;
+    }
+};
+
+void pipeline_outputs_match_operators()
+{
+    PipelineConfig config;
+    config.total_nantennas(16);
+    config.nchans(16);
+    config.cb_nantennas(16);
+    config.cb_nbeams(64);
+    config.cb_fscrunch(1);
+    config.cb_tscrunch(16);
+    config.ib_fscrunch(1);
+    config.cb_heap_size(2048);   // 128 output samples x 16 channels per heap
+    config.input_level(16.0f);
+    config.quantisation_mode(FBF_QUANT_ROUND);
+    config.centre_frequency(1284e6f);
+    config.bandwidth(856e6f);
+    config.delay_buffer_shm("fbfuse_test_pv_shm");
+    config.delay_buffer_mutex("fbfuse_test_pv_mutex");
+    config.delay_buffer_sem("fbfuse_test_pv_count");
+    DelayEngineSimulator simulator(config);
+    const std::size_t nd = config.cb_nbeams() * config.cb_nantennas();
+    std::vector<float> offs(nd);
+    FBF_ERROR_CHECK(fbf_synth_uniform_f32(77, -1e-6f, 1e-6f, offs.data(), nd));
+    for (std::size_t i = 0; i < nd; ++i) simulator.delays()[i] = float2{1e-9f, offs[i]};
+    simulator.update_delays();
+
+    const int nhg = 8;
+    const std::size_t block_bytes = (std::size_t)nhg * 16 * 16 * 256 * 2 * 2;
+    const std::size_t cb_bytes = 64 * (nhg * 256 / 16) * 16;
+    const std::size_t ib_bytes = (nhg * 256 / 16) * 16;
+    DadaDB cb_buffer(16, cb_bytes, 4, 4096), ib_buffer(16, ib_bytes, 4, 4096);
+    cb_buffer.create();
+    ib_buffer.create();
+    MultiLog log("PipelineValue");
+    DadaWriteClient cbw(cb_buffer.key(), log), ibw(ib_buffer.key(), log);
+    const int nblocks = 6;
+    std::vector<std::vector<char2>> inputs;
+    for (int i = 0; i < nblocks; ++i) inputs.push_back(gaussian_char2(5000 + i, 16.0, block_bytes / 2, true));
+    CaptureSink cb_sink;
+    long double ts[nblocks];
+    {
+        Pipeline pipeline(config, cbw, ibw, block_bytes);
+        std::vector<char> hbuf(4096, 0);
+        RawBytes hrb(hbuf.data(), 4096, 4096);
+        Header header(hrb);
+        header.set<long double>("SAMPLE_CLOCK", 1712000000.0L);
+        header.set<long double>("SYNC_TIME", 1600000000.0L);
+        header.set<std::size_t>("SAMPLE_CLOCK_START", 123456);
+        pipeline.init(hrb);
+        void* in = nullptr;
+        FBF_ERROR_CHECK(fbf_host_malloc(&in, block_bytes));
+        RawBytes in_rb(static_cast<char*>(in), block_bytes, block_bytes);
+        for (int i = 0; i < nblocks; ++i) {
+            std::memcpy(in, inputs[i].data(), block_bytes);
+            pipeline(in_rb);
+            if (i >= 1) ts[i - 1] = pipeline.unix_timestamp();
+        }
+        FBF_ERROR_CHECK(fbf_host_free(in));
+    }   // destructor releases the last output block
+    {
+        DadaReadClient reader(cb_buffer.key(), log);
+        auto& h = reader.header_stream().next();
+        cb_sink.init(h);
+        reader.header_stream().release();
+        for (int i = 0; i < nblocks - 2; ++i) {
+            cb_sink(reader.data_stream().next());
+            reader.data_stream().release();
+        }
+    }
+    EXPECT(cb_sink.header.find("SAMPLE_CLOCK_START 123456") != std::string::npos);
+    EXPECT(cb_sink.blocks.size() == (std::size_t)(nblocks - 2));
+    // reference result per block through the operator classes
+    StreamType stream = nullptr;
+    FBF_ERROR_CHECK(fbf_stream_create(&stream));
+    SplitTranspose st(config);
+    CoherentBeamformer cb(config);
+    WeightsManager wm(config, stream);
+    std::vector<float2> dh(simulator.delays(), simulator.delays() + nd);
+    WeightsManager::DelayVectorType dd(dh);
+    for (int i = 0; i < nblocks - 2; ++i) {
+        SplitTranspose::VoltageType taftp(inputs[i]), ftpa;
+        st.transpose(taftp, ftpa, stream);
+        auto const& w = wm.weights(dd, (double)ts[i]);
+        CoherentBeamformer::PowerVectorType out;
+        cb.beamform(ftpa, w, out, stream);
+        std::vector<int8_t> expect = out.to_host();
+        EXPECT(expect.size() == cb_sink.blocks[i].size());
+        EXPECT(std::memcmp(expect.data(), cb_sink.blocks[i].data(), expect.size()) == 0);
+    }
+    // the epochs advance by one block of samples
+    EXPECT(std::fabs((double)(ts[1] - ts[0]) - 2.0 * 4096 * nhg * 256 / 1712e6) < 1e-9);
+    fbf_stream_destroy(stream);
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -262,6 +559,12 @@ int main(int argc, char** argv)
         {"WeightsManagerTester.test_real_value", weights_real_value},
         {"DoubleDeviceBuffer.semantics", double_device_buffer_semantics},
         {"CoherentBeamformer.error_paths_throw", error_paths_throw},
+        {"Header.get_set", header_get_set},
+        {"DadaRing.write_read_eod", dada_ring_write_read_eod},
+        {"DelayManagerTester.test_updates", delay_manager_updates},
+        {"DelayManagerTester.test_bad_keys", delay_manager_bad_keys},
+        {"PipelineTester.simple_run_test", pipeline_simple_run},
+        {"Pipeline.outputs_match_operators", pipeline_outputs_match_operators},
     };
     int ran = 0;
     for (auto& c : cases) {

@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-channels", type=int, default=64, help="channels in the CPU sample (64 = whole block)")
     ap.add_argument("--variant", type=int, default=0, help="CB kernel variant (0 = auto)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 40/16/8 subarray leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the four-feed serving leg")
+    ap.add_argument("--serving-seconds", type=float, default=8.0, help="Poisson arrival window of the serving leg")
+    ap.add_argument("--hbm-budget-gb", type=float, default=64.0, help="serving engine HBM budget")
     return ap.parse_args()
 
 
@@ -188,10 +192,91 @@ def main():
         },
     }
 
+    wp.close()
+    cb.close()
+    ib.close()
+    for b in (d_taftp, d_ftpa, d_delays, d_w, d_cb, d_ib):
+        b.free()
+    if not args.no_batch:
+        result["batched_subarrays"] = batch_leg(min(K, 100), args.warmup)
+    if not args.no_serving:
+        result["serving"] = serving_leg(args.serving_seconds, int(args.hbm_budget_gb * 1e9))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(cfg, sub, taftp, args.cpu_channels)
     print(json.dumps(result), flush=True)
     return 0
+
+
+def batch_leg(K, warmup):
+    """One 64-antenna block split 40/16/8 antennas with 768/256/64 beams, plus
+    the incoherent beam, as one captured hipGraph per block; kernel variants
+    and chunk lengths picked by the autotuner, for tscrunch 4, 16 and 64."""
+    from psrdada_cpp_amd import capi, workloads as wl
+
+    out = {}
+    for ts in (4, 16, 64):
+        cfg = wl.config3(ts)
+        subs = [dict(start_antenna=sb.start_antenna, nantennas=sb.nantennas, nbeams=sb.nbeams, tscrunch=ts,
+                     fscrunch=1, nsamples_per_heap_out=cfg.nsamples // ts) for sb in cfg.subarrays]
+        levels = [cfg.levels(sb)[0] for sb in cfg.subarrays]
+        batch = capi.Batch(cfg.total_nantennas, cfg.nchans, cfg.nheap_groups, subs, levels)
+        (_, _), (ibo, ibs) = cfg.levels(cfg.subarrays[0])
+        batch.enable_ib(cfg.ib_tscrunch, cfg.ib_fscrunch, ibo, ibs)
+        t_tune = time.perf_counter()
+        batch.autotune(reps=3)
+        t_tune = time.perf_counter() - t_tune
+        s = capi.Stream()
+        d_taftp = capi.DeviceBuffer.from_array(wl.synth_taftp(cfg, 31 + ts), s)
+        ws = [capi.DeviceBuffer.from_array(capi.synth_gaussian_int8(40 + i, 60.0, batch.weights_bytes(i)), s)
+              for i in range(len(subs))]
+        outs = [capi.DeviceBuffer(batch.output_bytes(i)) for i in range(len(subs) + 1)]
+        for _ in range(warmup):
+            batch.run(d_taftp, ws, outs, use_graph=True, stream=s)
+        s.sync()
+        e0, e1 = capi.Event(), capi.Event()
+        t0 = time.perf_counter()
+        e0.record(s)
+        for _ in range(K):
+            batch.run(d_taftp, ws, outs, use_graph=True, stream=s)
+        e1.record(s)
+        s.sync()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / K
+        dev_ms = e0.elapsed_ms(e1) / K
+        ops = cfg.cb_ops()
+        out[f"tscrunch_{ts}"] = {
+            "ms_per_block": round(wall_ms, 4),
+            "device_ms_per_block": round(dev_ms, 4),
+            "x_real_time": round(cfg.block_seconds * 1e3 / wall_ms, 2),
+            "int8_tops": round(ops / (wall_ms * 1e-3) / 1e12, 2),
+            "autotune_s": round(t_tune, 2),
+        }
+        batch.close()
+        for b in (d_taftp, *ws, *outs):
+            b.free()
+    out["workload"] = "64-ant block split 40/16/8 ant with 768/256/64 beams + IB, one hipGraph per block"
+    return out
+
+
+def serving_leg(seconds, budget):
+    """Four feeds (64/768/64, 32/256/64, 16/128/32, 8/64/16 ant/beams/ch),
+    Poisson arrivals at 1.5x real time per feed from pinned host memory, then
+    closed-loop saturation with host and with device-resident input."""
+    from psrdada_cpp_amd import serving
+
+    feeds = serving.four_subarray_feeds(32)
+    srv, fids = serving.make_server(feeds, budget, nslots=8)
+    try:
+        out = {
+            "workload": "4 feeds 64/768/64, 32/256/64, 16/128/32, 8/64/16 (ant/beams/ch), 32 heap groups, "
+                        "ts 16 + IB; continuous batching over 8 streams",
+            "poisson_1.5x": serving.run_poisson(srv, fids, feeds, 1.5, seconds),
+            "saturation_host_input": serving.run_saturation(srv, fids, feeds, 24),
+            "saturation_device_input": serving.run_saturation(srv, fids, feeds, 64, device_input=True),
+        }
+    finally:
+        srv.close()
+    out["block_ms"] = round(feeds[0].block_seconds * 1e3, 3)
+    return out
 
 
 def cpu_baseline(cfg, sub, taftp, nch):

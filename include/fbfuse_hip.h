@@ -246,6 +246,83 @@ int fbf_batch_autotune(fbf_batch* batch, int reps);
 int fbf_batch_run(fbf_batch* batch, const void* taftp, size_t taftp_bytes,
                   const void* const* weights, void* const* outputs, int use_graph, fbf_stream_t stream);
 
+/* ---- serving engine (no counterpart in the reference, whose Pipeline
+ * processes one subarray block by block, Pipeline.cu:173-247) -------------
+ * Feeds are subarrays with their own geometry, plans and device-resident
+ * weights.  Blocks submitted for any feed wait in one FIFO; a dispatcher
+ * admits the oldest block as soon as a slot (a HIP stream) is free and its
+ * buffers fit in the HBM budget, so blocks that arrive while others run are
+ * admitted as capacity frees up and their work overlaps on the device.  Per
+ * block: H2D copy (unless the input is already on the device), split
+ * transpose, coherent beams, optional incoherent beam, D2H copy of the beams
+ * into the caller's host buffers; completions are reported in finishing
+ * order with their latency (submission to completion). */
+typedef struct fbf_server fbf_server;
+
+typedef struct fbf_feed_desc {
+    int nantennas;              /* antennas of this feed's TAFTP blocks (all beamformed) */
+    int nbeams;
+    int nchans;
+    int nheap_groups;           /* heap groups per block */
+    int nsamples_per_heap;      /* 256 */
+    int tscrunch;
+    int fscrunch;
+    int nsamples_per_heap_out;  /* 0 -> 8192 / (F / fscrunch) */
+    int quant_mode;
+    int ib_tscrunch;            /* 0: no incoherent beam */
+    int ib_fscrunch;
+} fbf_feed_desc;
+
+typedef struct fbf_server_stats {
+    uint64_t submitted;
+    uint64_t completed;
+    uint64_t failed;
+    uint64_t in_flight;
+    uint64_t pending;
+    double p50_ms;              /* latency percentiles over completed blocks */
+    double p99_ms;
+    double max_ms;
+    double mean_ms;
+    double input_bytes;         /* bytes of blocks completed */
+    double wall_s;              /* seconds since fbf_server_start */
+    size_t hbm_budget;
+    size_t hbm_peak;            /* high-water mark of admitted block buffers */
+} fbf_server_stats;
+
+/* nslots concurrent blocks at most; hbm_budget bytes of device memory for
+ * block buffers (allocated once, at fbf_server_start). */
+int fbf_server_create(int nslots, size_t hbm_budget, fbf_server** server);
+int fbf_server_destroy(fbf_server* server);
+int fbf_server_add_feed(fbf_server* server, const fbf_feed_desc* desc,
+                        const float* cb_offsets, const float* cb_scales,
+                        const float* ib_offsets, const float* ib_scales, int* feed);
+int fbf_server_feed_bytes(const fbf_server* server, int feed, size_t* block_bytes,
+                          size_t* cb_bytes, size_t* ib_bytes);
+/* Device-resident weights: copied from host memory (FBA char2) ... */
+int fbf_server_set_weights(fbf_server* server, int feed, const void* weights_host, size_t bytes);
+/* ... or generated on the device from delays ([B][A] float2, host) at
+ * `epoch` for channel centres chan_freqs (host, nchans doubles).  Blocks
+ * admitted after the call use the new weights; blocks already running keep
+ * the old ones. */
+int fbf_server_set_delays(fbf_server* server, int feed, const void* delays_host, size_t bytes,
+                          const double* chan_freqs, double epoch);
+int fbf_server_start(fbf_server* server);
+/* Queues one block.  input: host (pinned for full speed) or device memory
+ * (input_on_device != 0) of fbf_server_feed_bytes' block size; cb_out/ib_out:
+ * host buffers for the beams (ib_out may be NULL without an incoherent beam).
+ * Buffers must stay valid until the block completes.  Returns
+ * FBF_ERR_CAPACITY if max_pending blocks are already queued. */
+int fbf_server_submit(fbf_server* server, int feed, const void* input, size_t bytes, int input_on_device,
+                      void* cb_out, void* ib_out, uint64_t tag);
+/* Next completed block: waits up to timeout_ms (0 = no wait).  Returns
+ * FBF_OK with *tag, *latency_ms and *status (FBF_OK or the block's error), or
+ * FBF_ERR_STATE if nothing completed in time. */
+int fbf_server_poll(fbf_server* server, uint64_t* tag, int* status, double* latency_ms, int timeout_ms);
+int fbf_server_stats_get(fbf_server* server, fbf_server_stats* stats);
+int fbf_server_reset_stats(fbf_server* server);
+/* Waits until every submitted block has completed. */
+int fbf_server_drain(fbf_server* server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,21 @@ class SubarrayDesc(C.Structure):
                 ("quant_mode", C.c_int)]
 
 
+class FeedDesc(C.Structure):
+    _fields_ = [("nantennas", C.c_int), ("nbeams", C.c_int), ("nchans", C.c_int), ("nheap_groups", C.c_int),
+                ("nsamples_per_heap", C.c_int), ("tscrunch", C.c_int), ("fscrunch", C.c_int),
+                ("nsamples_per_heap_out", C.c_int), ("quant_mode", C.c_int), ("ib_tscrunch", C.c_int),
+                ("ib_fscrunch", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("in_flight", C.c_uint64), ("pending", C.c_uint64), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("max_ms", C.c_double), ("mean_ms", C.c_double),
+                ("input_bytes", C.c_double), ("wall_s", C.c_double), ("hbm_budget", C.c_size_t),
+                ("hbm_peak", C.c_size_t)]
+
+
 class IbDesc(C.Structure):
     _fields_ = [("nantennas", C.c_int), ("nchans", C.c_int), ("nsamples_per_heap", C.c_int),
                 ("tscrunch", C.c_int), ("fscrunch", C.c_int), ("quant_mode", C.c_int)]
@@ -106,6 +121,18 @@ _SIGS = {
     "fbf_batch_set_variant": [vp, i32, i32, i32],
     "fbf_batch_autotune": [vp, i32],
     "fbf_batch_run": [vp, vp, sz, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), i32, vp],
+    "fbf_server_create": [i32, sz, pvp],
+    "fbf_server_destroy": [vp],
+    "fbf_server_add_feed": [vp, C.POINTER(FeedDesc), vp, vp, vp, vp, C.POINTER(C.c_int)],
+    "fbf_server_feed_bytes": [vp, i32, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "fbf_server_set_weights": [vp, i32, vp, sz],
+    "fbf_server_set_delays": [vp, i32, vp, sz, vp, C.c_double],
+    "fbf_server_start": [vp],
+    "fbf_server_submit": [vp, i32, vp, sz, i32, vp, vp, C.c_uint64],
+    "fbf_server_poll": [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_double), i32],
+    "fbf_server_stats_get": [vp, C.POINTER(ServerStats)],
+    "fbf_server_reset_stats": [vp],
+    "fbf_server_drain": [vp],
     "fbf_ib_plan_create": [C.POINTER(IbDesc), vp, vp, pvp],
     "fbf_ib_plan_set_levels": [vp, vp, vp],
     "fbf_ib_plan_destroy": [vp],
@@ -487,6 +514,105 @@ class Batch:
     def close(self):
         if self.handle:
             call("fbf_batch_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HostBuffer:
+    """Pinned host allocation exposed as a numpy array."""
+
+    def __init__(self, nbytes, dtype=np.int8):
+        self.nbytes = int(nbytes)
+        h = C.c_void_p()
+        call("fbf_host_malloc", C.byref(h), self.nbytes)
+        self.ptr = h
+        dtype = np.dtype(dtype)
+        buf = (C.c_char * self.nbytes).from_address(h.value)
+        self.array = np.frombuffer(buf, dtype=dtype, count=self.nbytes // dtype.itemsize)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            call("fbf_host_free", self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving engine (fbf_server): feeds, continuous batching, latency stats."""
+
+    def __init__(self, nslots, hbm_budget):
+        h = C.c_void_p()
+        call("fbf_server_create", int(nslots), int(hbm_budget), C.byref(h))
+        self.handle = h
+
+    def add_feed(self, nantennas, nbeams, nchans, nheap_groups, tscrunch, fscrunch, cb_levels,
+                 ib=None, ib_levels=None, nsamples_per_heap_out=0, quant_mode=QUANT_ROUND):
+        d = FeedDesc(nantennas, nbeams, nchans, nheap_groups, 256, tscrunch, fscrunch, nsamples_per_heap_out,
+                     quant_mode, ib[0] if ib else 0, ib[1] if ib else 0)
+        co = np.ascontiguousarray(cb_levels[0], np.float32)
+        cs = np.ascontiguousarray(cb_levels[1], np.float32)
+        io = np.ascontiguousarray(ib_levels[0], np.float32) if ib else None
+        is_ = np.ascontiguousarray(ib_levels[1], np.float32) if ib else None
+        f = C.c_int()
+        call("fbf_server_add_feed", self.handle, C.byref(d), _np_ptr(co), _np_ptr(cs),
+             _np_ptr(io) if io is not None else None, _np_ptr(is_) if is_ is not None else None, C.byref(f))
+        return f.value
+
+    def feed_bytes(self, feed):
+        a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        call("fbf_server_feed_bytes", self.handle, feed, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def set_weights(self, feed, weights):
+        w = np.ascontiguousarray(weights)
+        call("fbf_server_set_weights", self.handle, feed, _np_ptr(w), w.nbytes)
+
+    def set_delays(self, feed, delays, chan_freqs, epoch):
+        d = np.ascontiguousarray(delays, np.float32)
+        fr = np.ascontiguousarray(chan_freqs, np.float64)
+        call("fbf_server_set_delays", self.handle, feed, _np_ptr(d), d.nbytes, _np_ptr(fr), float(epoch))
+
+    def start(self):
+        call("fbf_server_start", self.handle)
+
+    def submit(self, feed, inp, cb_out, ib_out=None, tag=0, on_device=False):
+        """inp: HostBuffer or DeviceBuffer (on_device=True); cb_out/ib_out: HostBuffer."""
+        call("fbf_server_submit", self.handle, feed, inp.ptr, inp.nbytes, int(bool(on_device)), cb_out.ptr,
+             ib_out.ptr if ib_out is not None else None, C.c_uint64(tag))
+
+    def poll(self, timeout_ms=1000):
+        t, st, lat = C.c_uint64(), C.c_int(), C.c_double()
+        r = lib().fbf_server_poll(self.handle, C.byref(t), C.byref(st), C.byref(lat), int(timeout_ms))
+        if r == FBF_ERR_STATE:
+            return None
+        check(r)
+        return t.value, st.value, lat.value
+
+    def stats(self):
+        st = ServerStats()
+        call("fbf_server_stats_get", self.handle, C.byref(st))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        call("fbf_server_reset_stats", self.handle)
+
+    def drain(self):
+        call("fbf_server_drain", self.handle)
+
+    def close(self):
+        if self.handle:
+            call("fbf_server_destroy", self.handle)
             self.handle = None
 
     def __del__(self):

@@ -56,8 +56,11 @@ extern "C" int fbf_device_name(char* buf, size_t buflen)
     FBF_HIP_TRY(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     FBF_HIP_TRY(hipGetDeviceProperties(&prop, dev));
-    std::string s = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
-                    std::to_string(prop.multiProcessorCount) + " CUs)";
+    // The marketing name can be empty (it comes from the driver's product
+    // table); the architecture name and CU count are always there.
+    std::string s = std::string(prop.gcnArchName) + ", " + std::to_string(prop.multiProcessorCount) + " CUs, " +
+                    std::to_string(prop.totalGlobalMem >> 30) + " GiB";
+    if (prop.name[0]) s = std::string(prop.name) + " (" + s + ")";
     std::strncpy(buf, s.c_str(), buflen - 1);
     buf[buflen - 1] = 0;
     return FBF_OK;

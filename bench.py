@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the four-feed serving leg")
     ap.add_argument("--serving-seconds", type=float, default=8.0, help="Poisson arrival window of the serving leg")
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0, help="serving engine HBM budget")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 3 times (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -56,6 +58,9 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
                                    "(the path shards by frequency channel with no exchange, see DESIGN.md)"}))
         return 2
+
+    if args.calibrate_copy:
+        return calibrate_copy()
 
     from psrdada_cpp_amd import capi, workloads as wl
     import ctypes
@@ -204,6 +209,23 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(cfg, sub, taftp, args.cpu_channels)
     print(json.dumps(result), flush=True)
+    return 0
+
+
+def calibrate_copy():
+    """A plain elementwise copy (int32 negate) of exactly 1 GiB in and 1 GiB
+    out, launched 3 times: what the HBM byte counters read for it converts
+    their units into bytes."""
+    import torch
+
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy": {"kernel": "torch.neg int32", "read_bytes": 4 * n, "write_bytes": 4 * n,
+                                         "launches": 3}}), flush=True)
     return 0
 
 

@@ -139,10 +139,24 @@ int fbf_cb_plan_destroy(fbf_cb_plan* plan);
 int fbf_cb_plan_set_variant(fbf_cb_plan* plan, int variant);
 int fbf_cb_variant_count(int* count);
 int fbf_cb_output_bytes(const fbf_cb_plan* plan, size_t ftpa_bytes, size_t* tbtf_bytes);
-/* FTPA + FBA -> int8 TBTF. */
+/* FTPA + FBA -> int8 TBTF.  The int8 beams pass through a staging buffer
+ * ([Fo][To][B], the size of the output) that the plan owns and grows on
+ * demand, so calls with one plan on different streams must not overlap, and
+ * a stream that is being captured needs fbf_cb_plan_reserve first. */
 int fbf_cb_beamform(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
                     const void* fba_weights, size_t weights_bytes,
                     void* tbtf, size_t tbtf_bytes, fbf_stream_t stream);
+/* As fbf_cb_beamform with a caller-owned staging buffer of at least
+ * fbf_cb_output_bytes bytes (16-byte aligned): one plan can then run on
+ * several streams at once.  No counterpart in the reference; used by the
+ * serving engine. */
+int fbf_cb_beamform_ws(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
+                       const void* fba_weights, size_t weights_bytes,
+                       void* tbtf, size_t tbtf_bytes, void* workspace, size_t workspace_bytes,
+                       fbf_stream_t stream);
+/* Sizes the plan's staging buffer for blocks of nsamples samples (call before
+ * capturing a stream that runs the plan). */
+int fbf_cb_plan_reserve(fbf_cb_plan* plan, long long nsamples);
 /* Beamform the plan's antennas as a subarray of a full-antenna FTPA block:
  * ftpa_block holds block_nantennas antennas per column and this subarray's
  * antennas start at antenna_offset.  Same output as splitting the subset

@@ -25,10 +25,13 @@ CPP_LIB = os.path.join(PKG, "libfbfuse_cpp.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-# Packed FP32 instructions are disabled for all device code.
+# Packed FP32 instructions are disabled for all device code.  MFMA results
+# go to ordinary VGPRs (gfx950 reads them directly), which saves one
+# v_accvgpr_read per accumulator value in the detection loops.
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall", "-Wno-unused-result",
 ]
 

@@ -121,6 +121,8 @@ _SIGS = {
     "fbf_batch_set_variant": [vp, i32, i32, i32],
     "fbf_batch_autotune": [vp, i32],
     "fbf_batch_run": [vp, vp, sz, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), i32, vp],
+    "fbf_cb_beamform_ws": [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp],
+    "fbf_cb_plan_reserve": [vp, C.c_longlong],
     "fbf_server_create": [i32, sz, pvp],
     "fbf_server_destroy": [vp],
     "fbf_server_add_feed": [vp, C.POINTER(FeedDesc), vp, vp, vp, vp, C.POINTER(C.c_int)],

@@ -87,6 +87,12 @@ extern "C" int fbf_batch_create(int total_nantennas, int nchans, int nsamples_pe
             return st;
         }
         b->plans.push_back(p);
+        // The graph captures the plan's staging buffer: size it now.
+        st = fbf_cb_plan_reserve(p, (long long)nsamples_per_heap * nheap_groups);
+        if (st) {
+            batch_release(b);
+            return st;
+        }
     }
     b->ftpa_bytes = (size_t)4 * total_nantennas * nchans * nsamples_per_heap * nheap_groups;
     hipError_t e = hipMalloc(&b->ftpa, b->ftpa_bytes);

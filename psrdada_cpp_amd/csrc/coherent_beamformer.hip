@@ -6,6 +6,13 @@
 //   P(b, fo, to)     = sum over fscrunch, tscrunch, pol of |beam|^2
 //   out              = quantise(P, offset[fo], scale[fo])   written TBTF
 //
+// The int8 output leaves the beamformer kernel as [Fo][To][B] (beams
+// contiguous: each work-group writes whole 64-byte runs) and a second kernel
+// (tbtf_k) moves it to TBTF [To/ohs][B][ohs][Fo].  Writing TBTF directly from
+// the beamformer scatters single bytes 1 channel apart across work-groups
+// (one work-group owns one channel), which the memory system turns into
+// 32-byte partial writes: 32x the output size in HBM writes.
+//
 // MI355X design.  Per channel the contraction is a real GEMM
 //   C[2B x T*P] = W'[2B x 2A] * V'[2A x T*P]
 // with the [Re, -Im; Im, Re] embedding: row Re_b = [wr_a, -wi_a] and
@@ -37,6 +44,12 @@ namespace fbf {
 namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Two adjacent beams' int8 outputs as one 16-bit store (b is even, B % 8 == 0).
+__device__ __forceinline__ void store_pair(int8_t* dst, int8_t a, int8_t b)
+{
+    *reinterpret_cast<uint16_t*>(dst) = (uint16_t)((uint8_t)a | ((uint16_t)(uint8_t)b << 8));
+}
 
 enum { MODE_INT8 = 0, MODE_SUMS = 1, MODE_POWER = 2 };
 
@@ -109,6 +122,20 @@ __device__ __forceinline__ uint32_t neg_odd(uint32_t x)
     return (x & 0x00FF00FFu) | (sat_neg_byte(x >> 8) << 8) | (sat_neg_byte(x >> 24) << 24);
 }
 
+// Embedded 16-byte weight slice of an Im row (swap_pairs) or a Re row
+// (neg_odd), without branches: both forms are computed and a bit mask picks.
+__device__ __forceinline__ v4i embed_slice(v4i raw, bool im_row)
+{
+    const uint32_t m = im_row ? 0xFFFFFFFFu : 0u;
+    v4i e;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t x = (uint32_t)raw[q];
+        e[q] = (int)((swap_pairs(x) & m) | (neg_odd(x) & ~m));
+    }
+    return e;
+}
+
 template <int ROWT, int KSTEPS, int MODE, bool AL16>
 __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
 {
@@ -149,15 +176,13 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
             const bool bvalid = b < p.B;
             const int8_t* wcol = p.fba + ((size_t)f * p.B + (bvalid ? b : p.B - 1)) * K;
 #pragma unroll
-            for (int s = 0; s < KSTEPS; ++s) {
-                const v4i raw = load_wslice<AL16>(wcol, s * 64 + g * 16, K, bvalid);
-                v4i e;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    e[q] = (int)((c & 1) ? swap_pairs((uint32_t)raw[q]) : neg_odd((uint32_t)raw[q]));
-                w[r][s] = e;
-            }
+            for (int s = 0; s < KSTEPS; ++s) w[r][s] = load_wslice<AL16>(wcol, s * 64 + g * 16, K, bvalid);
         }
+        // Every load is in flight before the first is used.
+#pragma unroll
+        for (int r = 0; r < ROWT; ++r)
+#pragma unroll
+            for (int s = 0; s < KSTEPS; ++s) w[r][s] = embed_slice(w[r][s], c & 1);
 
         const int8_t* vcol = p.ftpa + ((size_t)f * TP + (size_t)t_addr * tsp) * p.vstride + p.voff;
         v4i v[KSTEPS];
@@ -209,19 +234,16 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
         }
         return;
     }
-    // int8 TBTF: [to / ohs][b][to % ohs][fo]
+    // int8, staged [fo][to][b]
     const float off = p.offsets[fo];
     const float sc = p.scales[fo];
-    int8_t* out = static_cast<int8_t*>(p.out);
-    const size_t tf_size = (size_t)p.ohs * p.Fo;
-    const size_t heap_base = (size_t)(t_out / p.ohs) * p.B * tf_size + (size_t)(t_out % p.ohs) * p.Fo + fo;
+    int8_t* row = static_cast<int8_t*>(p.out) + ((size_t)fo * p.nsamples_out + t_out) * p.B;
 #pragma unroll
     for (int r = 0; r < ROWT; ++r) {
         const int b0 = beam_base + r * 8 + 2 * g;
-        if (b0 < p.B) {
-            out[heap_base + (size_t)b0 * tf_size] = quantise((double)pw[r][0], off, sc, p.quant);
-            out[heap_base + (size_t)(b0 + 1) * tf_size] = quantise((double)pw[r][1], off, sc, p.quant);
-        }
+        if (b0 < p.B)
+            store_pair(row + b0, quantise((double)pw[r][0], off, sc, p.quant),
+                       quantise((double)pw[r][1], off, sc, p.quant));
     }
 }
 
@@ -382,15 +404,13 @@ __global__ __launch_bounds__(256) void cb_mfma32_k(CbArgs p)
             const bool bvalid = b < p.B;
             const int8_t* wcol = p.fba + ((size_t)f * p.B + (bvalid ? b : p.B - 1)) * K;
 #pragma unroll
-            for (int s = 0; s < KSTEPS; ++s) {
-                const v4i raw = load_wslice<AL16>(wcol, 16 * (h * KSTEPS + s), K, bvalid);
-                v4i e;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    e[q] = (int)((c & 1) ? swap_pairs((uint32_t)raw[q]) : neg_odd((uint32_t)raw[q]));
-                w[r][s] = e;
-            }
+            for (int s = 0; s < KSTEPS; ++s) w[r][s] = load_wslice<AL16>(wcol, 16 * (h * KSTEPS + s), K, bvalid);
         }
+        // Every load is in flight before the first is used.
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int s = 0; s < KSTEPS; ++s) w[r][s] = embed_slice(w[r][s], c & 1);
 
         const int8_t* vcol = p.ftpa + ((size_t)f * TP + (size_t)t_addr * tsp) * p.vstride + p.voff;
         const long long n0 = t_out * tsp;
@@ -428,20 +448,18 @@ __global__ __launch_bounds__(256) void cb_mfma32_k(CbArgs p)
             }
         return;
     }
+    // int8, staged [fo][to][b]
     const float off = p.offsets[fo];
     const float sc = p.scales[fo];
-    int8_t* out = static_cast<int8_t*>(p.out);
-    const size_t tf_size = (size_t)p.ohs * p.Fo;
-    const size_t heap_base = (size_t)(t_out / p.ohs) * p.B * tf_size + (size_t)(t_out % p.ohs) * p.Fo + fo;
+    int8_t* row = static_cast<int8_t*>(p.out) + ((size_t)fo * p.nsamples_out + t_out) * p.B;
 #pragma unroll
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int b0 = beam_base + r * 16 + 4 * q + 2 * h;
-            if (b0 < p.B) {
-                out[heap_base + (size_t)b0 * tf_size] = quantise((double)pw[r][q][0], off, sc, p.quant);
-                out[heap_base + (size_t)(b0 + 1) * tf_size] = quantise((double)pw[r][q][1], off, sc, p.quant);
-            }
+            if (b0 < p.B)
+                store_pair(row + b0, quantise((double)pw[r][q][0], off, sc, p.quant),
+                           quantise((double)pw[r][q][1], off, sc, p.quant));
         }
 }
 
@@ -507,15 +525,13 @@ __global__ __launch_bounds__(256) void cb_lds_k(CbArgs p, int jc)
             const bool bvalid = b < p.B;
             const int8_t* wcol = p.fba + ((size_t)f * p.B + (bvalid ? b : p.B - 1)) * K;
 #pragma unroll
-            for (int s = 0; s < KSTEPS; ++s) {
-                const v4i raw = load_wslice<AL16>(wcol, 16 * (h * KSTEPS + s), K, bvalid);
-                v4i e;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    e[q] = (int)((c & 1) ? swap_pairs((uint32_t)raw[q]) : neg_odd((uint32_t)raw[q]));
-                w[r][s] = e;
-            }
+            for (int s = 0; s < KSTEPS; ++s) w[r][s] = load_wslice<AL16>(wcol, 16 * (h * KSTEPS + s), K, bvalid);
         }
+        // Every load is in flight before the first is used.
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int s = 0; s < KSTEPS; ++s) w[r][s] = embed_slice(w[r][s], c & 1);
 
         const int8_t* vbase = p.ftpa + (size_t)f * TP * p.vstride + p.voff;
         Piece pre[kLdsMaxPieces];
@@ -588,20 +604,18 @@ __global__ __launch_bounds__(256) void cb_lds_k(CbArgs p, int jc)
             }
         return;
     }
+    // int8, staged [fo][to][b]
     const float off = p.offsets[fo];
     const float sc = p.scales[fo];
-    int8_t* out = static_cast<int8_t*>(p.out);
-    const size_t tf_size = (size_t)p.ohs * p.Fo;
-    const size_t heap_base = (size_t)(t_out / p.ohs) * p.B * tf_size + (size_t)(t_out % p.ohs) * p.Fo + fo;
+    int8_t* row = static_cast<int8_t*>(p.out) + ((size_t)fo * p.nsamples_out + t_out) * p.B;
 #pragma unroll
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int b0 = beam_base + r * 16 + 4 * q + 2 * h;
-            if (b0 < p.B) {
-                out[heap_base + (size_t)b0 * tf_size] = quantise((double)pw[r][q][0], off, sc, p.quant);
-                out[heap_base + (size_t)(b0 + 1) * tf_size] = quantise((double)pw[r][q][1], off, sc, p.quant);
-            }
+            if (b0 < p.B)
+                store_pair(row + b0, quantise((double)pw[r][q][0], off, sc, p.quant),
+                           quantise((double)pw[r][q][1], off, sc, p.quant));
         }
 }
 
@@ -698,6 +712,625 @@ hipError_t launch_mode(const CbArgs& a, int rowt, int ksteps, dim3 grid, hipStre
     return al16 ? launch_rowt<4, MODE, true>(a, ksteps, grid, s) : launch_rowt<4, MODE, false>(a, ksteps, grid, s);
 }
 
+// ---------------------------------------------------------------------------
+// Transposed tiles: voltages as the A operand, weights as the B operand.
+//
+//   D[m][n] = sum_k V[m][k] W'[n][k]
+//
+// Rows m are 32 consecutive input columns (t, p) of one channel, columns n
+// are 16 beams x (Re, Im) embedding rows.  Lane l holds column n = l % 32
+// and rows 8*(i/4) + 4*(l/32) + i%4 (i = 0..15).  For an integration window
+// of W = 2*tscrunch = 8, 16 or 32 columns, or a multiple of 32, every window
+// a lane touches is a fixed group of its 16 values, so the squares sum into
+// one int64 per (beam tile, window) in the lane.  The four lanes holding
+// {Re, Im} x {row halves} of a beam are combined once per pass of windows by
+// a reduce-scatter (exchange with lane ^ 1, then lane ^ 32) that leaves each
+// lane a distinct (beam, window) total to quantise.
+//
+// A wave keeps RT beam tiles (16 RT beams) of weights in registers for its
+// channel and reads each voltage tile once from LDS for all of them: LDS
+// traffic is 1/RT of a tile per MFMA.  With fscrunch FS = 2 or 4 the waves of
+// a work-group take different channels of the fscrunch group (each with its
+// own weights) and add their window totals through LDS before quantising.
+//
+// Voltage staging is a ring of NB tile sets in LDS (one tile per channel of
+// the fscrunch group per set) filled by direct global->LDS loads issued
+// NB - 1 tiles ahead.  Every step issues one set (past the work-group's last
+// tile the loads repeat the last tile into a slot nobody reads), so the
+// number of loads in flight after a wave's loads of the current set is the
+// constant (NB - 2) * M and the wait is a fixed s_waitcnt.
+// ---------------------------------------------------------------------------
+
+// s_waitcnt vmcnt(N) lgkmcnt(0).
+template <int N>
+__device__ __forceinline__ void wait_vm_lgkm0()
+{
+    static_assert(N >= 0 && N <= 63, "vmcnt");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4));
+}
+
+// One 16-byte global->LDS load per lane: lane i's 16 bytes land at
+// lds_run + 16 * i (lds_run is wave-uniform).
+__device__ __forceinline__ void dma16(const int8_t* src, int8_t* lds_run)
+{
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_run, 16, 0, 0);
+}
+
+// Work-group barrier without the memory fence of __syncthreads (which would
+// wait for every outstanding global load, including the LDS prefetches).
+// Callers wait for their own LDS and global->LDS traffic first; the empty
+// asm statements keep the compiler from moving memory accesses across.
+__device__ __forceinline__ void block_barrier()
+{
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
+struct CbTrArgs {
+    CbArgs p;
+    int ntiles;                // 32-column tiles per channel
+    int tps;                   // tiles per window (W >= 32), else 1
+    int npasses;               // passes per work-group
+    int ntp;                   // tiles per pass
+    int n_bgroups, n_tgroups;
+};
+
+// Value of lane ^ 1 (DPP quad_perm [1, 0, 3, 2]).
+__device__ __forceinline__ long long xor1_64(long long v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(unsigned long long)v, 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)((unsigned long long)v >> 32), 0xB1, 0xF, 0xF, false);
+    return (long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// a + b' where lanes below 32 keep a and take b' = (a of lane + 32), and
+// lanes from 32 keep b and take (b of lane - 32): the halves' exchange of a
+// reduce-scatter in one v_permlane32_swap per dword.  With a = value for the
+// lower half and b = value for the upper half, every lane gets its own sum.
+__device__ __forceinline__ long long swap32_sum(long long a, long long b)
+{
+    const unsigned alo = (uint32_t)(unsigned long long)a, ahi = (uint32_t)((unsigned long long)a >> 32);
+    const unsigned blo = (uint32_t)(unsigned long long)b, bhi = (uint32_t)((unsigned long long)b >> 32);
+    const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+    const long long x = (long long)(((unsigned long long)hi[0] << 32) | lo[0]);
+    const long long y = (long long)(((unsigned long long)hi[1] << 32) | lo[1]);
+    return x + y;
+}
+
+// Quantiser without the IEEE division in the common case.  With
+// inv = RN(1 / scale), y = RN((P - offset) * inv) is within 2^-43 of the
+// exactly rounded quotient the oracle computes for |y| <= 256, so when y is
+// more than 2^-40 away from every rounding boundary (half-integers when
+// rounding, integers when truncating) it rounds to the same integer; values
+// beyond +-256 saturate either way.  Otherwise (and for a zero, subnormal or
+// non-finite scale) the exact quantise() runs.
+struct QuantConst {
+    float off, sc;
+    double off_d, inv;
+    bool fast;
+};
+
+__device__ __forceinline__ QuantConst quant_const(float off, float sc)
+{
+    QuantConst q;
+    q.off = off;
+    q.sc = sc;
+    q.off_d = (double)off;
+    q.inv = 1.0 / (double)sc;
+    const float a = fabsf(sc);
+    q.fast = a >= 1.17549435e-38f && a <= 3.40282347e+38f && fabs(q.inv) <= 1.79769e308;
+    return q;
+}
+
+__device__ __forceinline__ int8_t quantise_fast(long long P, const QuantConst& q, int mode)
+{
+    const double y = ((double)P - q.off_d) * q.inv;
+    const double r = rint(y);
+    const double dist = mode ? fabs(y - r) : fabs(fabs(y - floor(y)) - 0.5);
+    if (!q.fast || !(dist > 0x1p-40)) return quantise((double)P, q.off, q.sc, mode);
+    double k = mode ? trunc(y) : r;
+    k = fmin(fmax(k, -128.0), 127.0);
+    return (int8_t)(int)k;
+}
+
+// Staging constants of cb_tr_k for FS channels x KSTEPS 32-byte K steps.
+template <int FS, int KSTEPS, int NTS>
+struct TrStage {
+    static constexpr int NW = 4 * NTS;                     // waves per work-group
+    static constexpr int NI = NTS * FS * KSTEPS;           // 1 KiB loads per set
+    static constexpr int M = (NI + NW - 1) / NW;           // per wave (the rest load into a dummy run)
+    static constexpr int SET = NI * 1024;                  // bytes per set
+    static constexpr int NB = 60 / NI < 2 ? 2 : (60 / NI > 8 ? 8 : 60 / NI);
+    static constexpr int WAIT = (NB - 2) * M;
+    static constexpr int DUMMY = (NI % NW) ? NW * 1024 : 0;
+};
+
+template <int RT, int NSLOT>
+struct TrRed {
+    static constexpr int VALS = RT == 2 ? NSLOT / 2 : NSLOT;   // totals per lane after the reduce-scatter
+};
+
+// RT: beam tiles per wave; SPT: windows per tile (1, 2, 4); NSLOT: windows
+// per pass held in registers; FS: fscrunch; NTS: time slots per work-group.
+//
+// A work-group has 4 * NTS waves: NTS time slots (contiguous runs of passes
+// of the same channel group and beams) x 4 waves ((4 / FS) beam groups x FS
+// channels).  With NTS = 3 a work-group holds every wave of its compute unit
+// (three per SIMD), so the per-step barrier keeps them in lockstep: waves of
+// separate work-groups sharing a SIMD progress at different rates (the issue
+// arbiter prefers older waves), which leaves SIMDs with one wave at the end.
+template <int RT, int KSTEPS, int SPT, int NSLOT, int FS, int NTS>
+__global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
+{
+    using St = TrStage<FS, KSTEPS, NTS>;
+    constexpr int KPAD = 32 * KSTEPS;
+    constexpr int PR = 2 * KSTEPS;                 // 16-byte pieces per LDS column
+    constexpr int VALS = TrRed<RT, NSLOT>::VALS;
+    constexpr int NB = St::NB;
+    static_assert(RT == 2 || RT == 4, "RT");
+    static_assert(NSLOT % SPT == 0 && (RT == 4 || NSLOT % 2 == 0), "slots");
+    static_assert(St::M <= 2, "at most two staging loads per wave per set");
+    extern __shared__ __align__(16) int8_t lds[];
+    const CbArgs& p = ta.p;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int h = lane >> 5;
+    const int c = lane & 31;
+    const int part = c & 1;
+    const int tsl = wv >> 2;         // time slot
+    const int wc = (wv & 3) % FS;    // channel of the fscrunch group
+    const int wb = (wv & 3) / FS;    // beam group within the work-group
+    constexpr int NWB = 4 / FS;
+
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int bgroup = logical % ta.n_bgroups;
+    const int rest = logical / ta.n_bgroups;
+    const int tgroup = rest % ta.n_tgroups;
+    const int fo = rest / ta.n_tgroups;
+    if (fo >= p.Fo) return;   // uniform over the work-group
+
+    const int K = 2 * p.A;
+    const int f = fo * FS + wc;
+    const int beam_base = (bgroup * NWB + wb) * 16 * RT;
+    const size_t TP = (size_t)2 * p.nsamples;
+    const int slot_tiles = ta.npasses * ta.ntp;
+    const int tile0 = (tgroup * NTS + tsl) * slot_tiles;
+
+    // Weights (B operand): embedding row n = c of beam tile r.
+    v4i w[RT][KSTEPS];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        const int b = beam_base + r * 16 + (c >> 1);
+        const bool bvalid = b < p.B;
+        const int8_t* wcol = p.fba + ((size_t)f * p.B + (bvalid ? b : p.B - 1)) * K;
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) w[r][s] = load_wslice<true>(wcol, 16 * (h * KSTEPS + s), K, bvalid);
+    }
+    // Every load is in flight before the first is used.
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) w[r][s] = embed_slice(w[r][s], part);
+
+    // Staging.  A wave's 16-byte load lands at LDS offset 16 * lane of a
+    // 1 KiB run, so the bank swizzle is chosen on the global side: physical
+    // piece pp of column col holds K piece (pp + rot(col)) % PR with
+    // rot(col) = col * KPAD / 256.  K pieces past the column (k >= K) re-read
+    // piece 0: the weights there are zero.
+    const int kpieces = K / 16;
+    int8_t* dummy = lds + NB * St::SET;
+    long long* red = reinterpret_cast<long long*>(dummy + St::DUMMY);   // [waves][VALS][64]
+    // A set holds one tile per (time slot, channel): tile index
+    // tsl * FS + ch.  This lane's loads: source offset less the tile's
+    // column offset, the first tile of the slot they serve, LDS destination.
+    size_t goff[St::M];
+    int gtile[St::M];
+    int ldst[St::M];
+#pragma unroll
+    for (int m = 0; m < St::M; ++m) {
+        const int u = wv + St::NW * m;
+        if (u < St::NI) {
+            const int off = u * 1024 + lane * 16;
+            const int ti = off / (32 * KPAD);
+            const int rem = off - ti * (32 * KPAD);
+            const int col = rem / KPAD;
+            const int pp = (rem - col * KPAD) >> 4;
+            const int j = (pp + ((col * KPAD) >> 8)) % PR;
+            const int jj = j < kpieces ? j : 0;
+            goff[m] = ((size_t)(fo * FS + ti % FS) * TP + col) * p.vstride + p.voff + 16 * jj;
+            gtile[m] = (tgroup * NTS + ti / FS) * slot_tiles;
+            ldst[m] = u * 1024;
+        } else {
+            goff[m] = (size_t)(fo * FS) * TP * p.vstride + p.voff;
+            gtile[m] = 0;
+            ldst[m] = -1;
+        }
+    }
+    const size_t tile_stride = (size_t)32 * p.vstride;
+    auto issue = [&](int q, int slot) {
+        int8_t* set = lds + slot * St::SET;
+#pragma unroll
+        for (int m = 0; m < St::M; ++m) {
+            const int tt = gtile[m] + q < ta.ntiles ? gtile[m] + q : ta.ntiles - 1;
+            int8_t* dst = ldst[m] >= 0 ? set + ldst[m] : dummy + wv * 1024;
+            dma16(p.ftpa + (size_t)tt * tile_stride + goff[m], dst);
+        }
+    };
+    // Physical offsets of this lane's K pieces in its column.
+    int aoff[KSTEPS];
+    {
+        const int rot = (c * KPAD) >> 8;
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s)
+            aoff[s] = ((tsl * FS + wc) * 32 + c) * KPAD + 16 * ((h * KSTEPS + s - rot + 32 * PR) % PR);
+    }
+
+    const bool wave_active = beam_base < p.B;
+#pragma unroll
+    for (int q = 0; q < NB - 1; ++q) issue(q, q);
+    int step = 0;
+    int rs = 0;          // ring slot read this step
+    int is = NB - 1;     // ring slot filled this step
+
+    const QuantConst qz = quant_const(p.offsets[fo], p.scales[fo]);
+    for (int pass = 0; pass < ta.npasses; ++pass) {
+        long long pw[RT][NSLOT];
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int sl = 0; sl < NSLOT; ++sl) pw[r][sl] = 0;
+
+        // Tiles of the pass: NSLOT/SPT groups of tps tiles (tps = 1 unless
+        // a window spans several tiles).
+#pragma unroll
+        for (int g = 0; g < NSLOT / SPT; ++g) {
+            for (int x = 0; x < ta.tps; ++x) {
+                // This wave's loads of the current set are done once WAIT
+                // later loads remain; the barrier makes every wave's loads
+                // visible and frees the slot read in the previous step.
+                wait_vm_lgkm0<St::WAIT>();
+                block_barrier();
+                issue(step + NB - 1, is);
+                is = is + 1 == NB ? 0 : is + 1;
+                if (wave_active) {
+                    const int8_t* src = lds + rs * St::SET;
+                    v4i a[KSTEPS];
+#pragma unroll
+                    for (int s = 0; s < KSTEPS; ++s) a[s] = *reinterpret_cast<const v4i*>(src + aoff[s]);
+#pragma unroll
+                    for (int r = 0; r < RT; ++r) {
+                        v16i acc = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+                        for (int s = 0; s < KSTEPS; ++s)
+                            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], w[r][s], acc, 0, 0, 0);
+                        // One v_mad_i64_i32 per value straight into the
+                        // window's accumulator (the RT chains and the other
+                        // waves of the SIMD cover the dependency latency).
+#pragma unroll
+                        for (int q = 0; q < SPT; ++q) {
+                            constexpr int n = 16 / SPT;
+#pragma unroll
+                            for (int i = 0; i < n; ++i)
+                                pw[r][g * SPT + q] += (long long)acc[q * n + i] * acc[q * n + i];
+                        }
+                    }
+                }
+                rs = rs + 1 == NB ? 0 : rs + 1;
+                ++step;
+            }
+        }
+
+        // Reduce-scatter over {Re, Im} (lane ^ 1, DPP) and row halves
+        // (lane ^ 32, v_permlane32_swap): no LDS round trips.
+        long long tot[VALS];
+        if constexpr (RT == 2) {
+            // lane keeps r = part: sum with lane ^ 1, then slots 2j + h
+            long long t1[NSLOT];
+#pragma unroll
+            for (int sl = 0; sl < NSLOT; ++sl) {
+                const long long u0 = pw[0][sl] + xor1_64(pw[0][sl]);
+                const long long u1 = pw[1][sl] + xor1_64(pw[1][sl]);
+                t1[sl] = part ? u1 : u0;
+            }
+#pragma unroll
+            for (int j = 0; j < VALS; ++j) tot[j] = swap32_sum(t1[2 * j], t1[2 * j + 1]);
+        } else {
+            // lane keeps r = part + 2h
+#pragma unroll
+            for (int sl = 0; sl < NSLOT; ++sl) {
+                long long u[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) u[r] = pw[r][sl] + xor1_64(pw[r][sl]);
+                tot[sl] = swap32_sum(part ? u[1] : u[0], part ? u[3] : u[2]);
+            }
+        }
+
+        // Sum over the fscrunch channels (one per wave of a beam group).
+        if constexpr (FS > 1) {
+#pragma unroll
+            for (int v = 0; v < VALS; ++v) red[(wv * VALS + v) * 64 + lane] = tot[v];
+            wait_vm_lgkm0<63>();
+            block_barrier();
+#pragma unroll
+            for (int v = 0; v < VALS; ++v) {
+                long long sum = 0;
+#pragma unroll
+                for (int ch = 0; ch < FS; ++ch) sum += red[((tsl * 4 + wb * FS + ch) * VALS + v) * 64 + lane];
+                tot[v] = sum;
+            }
+            wait_vm_lgkm0<63>();
+            block_barrier();
+        }
+
+        // Quantise and store (staged [fo][to][b] int8); with FS > 1 the waves
+        // of a beam group share the totals out.
+        const int win0 = (tile0 + pass * ta.ntp) * SPT / ta.tps;
+        const int r = RT == 2 ? part : part + 2 * h;
+        const int b = beam_base + r * 16 + (c >> 1);
+#pragma unroll
+        for (int v = 0; v < VALS; ++v) {
+            if (FS > 1 && v % FS != wc) continue;
+            const int sl = RT == 2 ? 2 * v + h : v;
+            const long long to = (long long)win0 + sl;
+            if (!wave_active || b >= p.B || to >= p.nsamples_out) continue;
+            static_cast<int8_t*>(p.out)[((size_t)fo * p.nsamples_out + (size_t)to) * p.B + b] =
+                quantise_fast(tot[v], qz, p.quant);
+        }
+    }
+    // The loads past the last tile are still in flight: LDS must not be
+    // released before they land.
+    wait_vm_lgkm0<0>();
+}
+
+// Launch of cb_tr_k.  Supports 8 | A (16-byte K slices), A <= 128,
+// tscrunch a power of two >= 4 (window 8, 16, 32 or a multiple of 32
+// columns), fscrunch 1, 2 or 4 with fscrunch * ceil(A / 16) <= 8, int8
+// output.
+inline int cb_tr_nslot(int rt, int W)
+{
+    if (W == 8) return 4;
+    return rt == 2 ? 4 : 2;
+}
+
+inline int cb_tr_nb(int ni) { return 60 / ni < 2 ? 2 : (60 / ni > 8 ? 8 : 60 / ni); }
+
+// Time slots per work-group of the transposed kernel.
+constexpr int kTrNts = 3;
+
+inline size_t cb_tr_shm(int fs, int ksteps, int vals)
+{
+    const int nw = 4 * kTrNts;
+    const int ni = kTrNts * fs * ksteps;
+    return (size_t)cb_tr_nb(ni) * ni * 1024 + ((ni % nw) ? (size_t)nw * 1024 : 0) +
+           (fs > 1 ? (size_t)nw * vals * 64 * sizeof(long long) : 0);
+}
+
+inline bool cb_tr_supported(int A, int ts, int fs, int rt)
+{
+    if (A % 8 != 0 || A > 128) return false;
+    if (ts < 4 || (ts & (ts - 1)) != 0) return false;
+    if (!(fs == 1 || fs == 2 || fs == 4)) return false;
+    const int ksteps = (2 * A + 31) / 32;
+    if (rt * ksteps > 16) return false;
+    if (fs * ksteps > 8) return false;
+    const int nslot = cb_tr_nslot(rt, 2 * ts);
+    const int vals = rt == 2 ? nslot / 2 : nslot;
+    return cb_tr_shm(fs, ksteps, vals) <= 65536;
+}
+
+// Compute units of the current device (cached per device).
+inline int device_cu_count()
+{
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
+// One round of resident work-groups: every (channel group, beam group) gets
+// (resident work-groups) / (channel groups x beam groups) work-groups, each a
+// contiguous run of passes, so the per-work-group start-up (weights, the
+// first staged tiles) is paid once per resident slot and all work-groups end
+// together.
+template <int RT, int KSTEPS, int SPT, int NSLOT, int FS>
+hipError_t launch_tr_k(CbTrArgs a, int passes, hipStream_t s)
+{
+    constexpr int NTS = kTrNts;
+    const size_t shm = cb_tr_shm(FS, KSTEPS, TrRed<RT, NSLOT>::VALS);
+    static int occ = 0;
+    if (occ == 0) {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, cb_tr_k<RT, KSTEPS, SPT, NSLOT, FS, NTS>, 256 * NTS,
+                                                         shm) != hipSuccess ||
+            o < 1)
+            o = 1;
+        occ = o;
+    }
+    // Work-groups per (channel group, beam group): one round of resident
+    // work-groups where the problem allows; each covers NTS slots of
+    // npasses passes.
+    const long long units = (long long)a.p.Fo * a.n_bgroups;
+    long long per_unit = (long long)occ * device_cu_count() / units;
+    if (per_unit < 1) per_unit = 1;
+    const long long max_groups = (passes + NTS - 1) / NTS;
+    if (per_unit > max_groups) per_unit = max_groups;
+    a.npasses = (int)((passes + per_unit * NTS - 1) / (per_unit * NTS));
+    a.n_tgroups = (int)((passes + (long long)a.npasses * NTS - 1) / ((long long)a.npasses * NTS));
+    const long long nblocks = units * a.n_tgroups;
+    if (nblocks <= 0 || nblocks >= (1ll << 31)) return hipErrorInvalidConfiguration;
+    cb_tr_k<RT, KSTEPS, SPT, NSLOT, FS, NTS><<<dim3((unsigned)nblocks), 256 * NTS, shm, s>>>(a);
+    return hipGetLastError();
+}
+
+template <int RT, int SPT, int NSLOT, int FS>
+hipError_t launch_tr_ks(const CbTrArgs& a, int ksteps, int passes, hipStream_t s)
+{
+    if constexpr (FS * 2 <= 8) {
+        if (ksteps == 1) return launch_tr_k<RT, 1, SPT, NSLOT, FS>(a, passes, s);
+        if (ksteps == 2) return launch_tr_k<RT, 2, SPT, NSLOT, FS>(a, passes, s);
+    }
+    if constexpr (FS * 4 <= 8) {
+        if (ksteps == 3) return launch_tr_k<RT, 3, SPT, NSLOT, FS>(a, passes, s);
+        if (ksteps == 4) return launch_tr_k<RT, 4, SPT, NSLOT, FS>(a, passes, s);
+    }
+    if constexpr (FS == 1 && RT == 2) {
+        switch (ksteps) {
+        case 5: return launch_tr_k<RT, 5, SPT, NSLOT, FS>(a, passes, s);
+        case 6: return launch_tr_k<RT, 6, SPT, NSLOT, FS>(a, passes, s);
+        case 7: return launch_tr_k<RT, 7, SPT, NSLOT, FS>(a, passes, s);
+        case 8: return launch_tr_k<RT, 8, SPT, NSLOT, FS>(a, passes, s);
+        default: break;
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int RT, int FS>
+hipError_t launch_tr_w(const CbTrArgs& a, int W, int ksteps, int passes, hipStream_t s)
+{
+    constexpr int base = RT == 2 ? 4 : 2;
+    if (W == 8) return launch_tr_ks<RT, 4, 4, FS>(a, ksteps, passes, s);
+    if (W == 16) return launch_tr_ks<RT, 2, base, FS>(a, ksteps, passes, s);
+    return launch_tr_ks<RT, 1, base, FS>(a, ksteps, passes, s);
+}
+
+template <int RT>
+hipError_t launch_tr(const CbTrArgs& a, int fs, int W, int ksteps, int passes, hipStream_t s)
+{
+    if (fs == 1) return launch_tr_w<RT, 1>(a, W, ksteps, passes, s);
+    if (fs == 2) return launch_tr_w<RT, 2>(a, W, ksteps, passes, s);
+    return launch_tr_w<RT, 4>(a, W, ksteps, passes, s);
+}
+
+// ---------------------------------------------------------------------------
+// Staged [Fo][To][B] int8 -> TBTF [To/ohs][B][ohs][Fo]
+//
+// A work-group moves a tile of up to 64 beams x tt samples (within one output
+// heap) x up to 64 channels.  Loads are 8-byte runs of beams; the tile is
+// scattered into LDS as [t][b][f] (channel bytes contiguous) and written out
+// as 4-byte words of channels, so one wave stores whole rows of
+// tt * Fo contiguous bytes per beam.  LDS dword layout: b pitch BP (odd),
+// t pitch TP = 64 * BP + 16, so that the word reads of a wave (16 words of
+// one (t, b) row x 4 samples) fall in distinct banks.
+// ---------------------------------------------------------------------------
+constexpr int kTrBeams = 64;
+constexpr int kTrMaxChans = 64;
+constexpr int kTrLdsBudget = 40 * 1024;
+
+struct TbtfArgs {
+    const int8_t* src;
+    int8_t* dst;
+    long long To;
+    int B, Fo, ohs;
+    int tt, ft, bp, tp;           // tile samples / channels, LDS pitches (dwords)
+    int n_ttiles, n_ftiles, n_btiles;
+};
+
+__global__ __launch_bounds__(256) void tbtf_k(TbtfArgs a)
+{
+    extern __shared__ __align__(16) int8_t tr_lds[];
+    long long bid = blockIdx.x;
+    const int bt = (int)(bid % a.n_btiles);
+    bid /= a.n_btiles;
+    const int ftile = (int)(bid % a.n_ftiles);
+    bid /= a.n_ftiles;
+    const int ttile = (int)(bid % a.n_ttiles);
+    const long long heap = bid / a.n_ttiles;
+    const int b0 = bt * kTrBeams, f0 = ftile * a.ft, t0 = ttile * a.tt;
+    const int bv = min(kTrBeams, a.B - b0), fv = min(a.ft, a.Fo - f0), tv = min(a.tt, a.ohs - t0);
+    const long long tg0 = heap * a.ohs + t0;
+
+    // Load: (piece of 8 beams, t, f), pieces fastest.
+    const int nb8 = bv >> 3;
+    const int nload = fv * tv * nb8;
+    for (int i = threadIdx.x; i < nload; i += 256) {
+        const int pb = i % nb8;
+        const int rest = i / nb8;
+        const int t = rest % tv;
+        const int f = rest / tv;
+        const uint2 v = *reinterpret_cast<const uint2*>(a.src + ((size_t)(f0 + f) * a.To + tg0 + t) * a.B + b0 +
+                                                        8 * pb);
+        int8_t* l = tr_lds + ((size_t)t * a.tp + (size_t)(8 * pb) * a.bp) * 4 + f;
+        const size_t bstep = (size_t)a.bp * 4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            l[k * bstep] = (int8_t)(v.x >> (8 * k));
+            l[(k + 4) * bstep] = (int8_t)(v.y >> (8 * k));
+        }
+    }
+    __syncthreads();
+
+    // Store: (word of 4 channels, t, b), words fastest; a beam's tv rows of
+    // Fo channels are contiguous in TBTF when the tile holds every channel.
+    int8_t* out = a.dst + (((size_t)heap * a.B + b0) * a.ohs + t0) * a.Fo + f0;
+    const size_t brow = (size_t)a.ohs * a.Fo;
+    if ((a.Fo & 3) == 0) {
+        const int nw = fv >> 2;
+        const int nstore = bv * tv * nw;
+        for (int i = threadIdx.x; i < nstore; i += 256) {
+            const int w = i % nw;
+            const int rest = i / nw;
+            const int t = rest % tv;
+            const int b = rest / tv;
+            const uint32_t word =
+                *reinterpret_cast<const uint32_t*>(tr_lds + ((size_t)t * a.tp + (size_t)b * a.bp + w) * 4);
+            *reinterpret_cast<uint32_t*>(out + b * brow + (size_t)t * a.Fo + 4 * w) = word;
+        }
+    } else {
+        const int nstore = bv * tv * fv;
+        for (int i = threadIdx.x; i < nstore; i += 256) {
+            const int f = i % fv;
+            const int rest = i / fv;
+            const int t = rest % tv;
+            const int b = rest / tv;
+            out[b * brow + (size_t)t * a.Fo + f] = tr_lds[((size_t)t * a.tp + (size_t)b * a.bp) * 4 + f];
+        }
+    }
+}
+
+// Tile shape and grid for tbtf_k.
+TbtfArgs tbtf_args(const int8_t* src, int8_t* dst, long long To, int B, int Fo, int ohs)
+{
+    TbtfArgs a{};
+    a.src = src;
+    a.dst = dst;
+    a.To = To;
+    a.B = B;
+    a.Fo = Fo;
+    a.ohs = ohs;
+    a.ft = Fo < kTrMaxChans ? Fo : kTrMaxChans;
+    a.bp = (a.ft + 3) / 4 + 1;
+    if ((a.bp & 1) == 0) ++a.bp;
+    a.tp = kTrBeams * a.bp + 16;
+    int tt = 16;
+    while (tt > 1 && (size_t)tt * a.tp * 4 > (size_t)kTrLdsBudget) tt >>= 1;
+    a.tt = tt < ohs ? tt : ohs;
+    a.n_ttiles = (ohs + a.tt - 1) / a.tt;
+    a.n_ftiles = (Fo + a.ft - 1) / a.ft;
+    a.n_btiles = (B + kTrBeams - 1) / kTrBeams;
+    return a;
+}
+
+hipError_t launch_tbtf(const TbtfArgs& a, hipStream_t s)
+{
+    const long long heaps = a.To / a.ohs;
+    const long long nblocks = heaps * a.n_ttiles * a.n_ftiles * a.n_btiles;
+    if (nblocks <= 0) return hipSuccess;
+    const size_t shm = (size_t)a.tt * a.tp * 4;
+    tbtf_k<<<dim3((unsigned)nblocks), 256, shm, s>>>(a);
+    return hipGetLastError();
+}
+
 }  // namespace
 }  // namespace fbf
 
@@ -712,7 +1345,31 @@ struct fbf_cb_plan {
     int variant;
     int chunk;        // LDS chunk length in columns, 0 = automatic
     float* d_levels;  // device: offsets[Fo] then scales[Fo]
+    void* staging;    // [Fo][To][B] int8 between the beamformer and tbtf_k
+    size_t staging_bytes;
 };
+
+// Grows the plan's staging buffer (not while the stream is being captured).
+static int cb_staging(fbf_cb_plan* plan, size_t bytes, hipStream_t s, void** out)
+{
+    if (plan->staging_bytes < bytes) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        FBF_HIP_TRY(hipStreamIsCapturing(s, &cs));
+        FBF_REQUIRE(cs == hipStreamCaptureStatusNone, FBF_ERR_STATE,
+                    "fbf_cb: the plan's staging buffer must be sized (fbf_cb_plan_reserve) before stream capture");
+        // Earlier launches on any stream may still use the old buffer.
+        if (plan->staging) {
+            FBF_HIP_TRY(hipDeviceSynchronize());
+            FBF_HIP_TRY(hipFree(plan->staging));
+        }
+        plan->staging = nullptr;
+        plan->staging_bytes = 0;
+        FBF_HIP_TRY(hipMalloc(&plan->staging, bytes));
+        plan->staging_bytes = bytes;
+    }
+    *out = plan->staging;
+    return FBF_OK;
+}
 
 static int cb_validate(const fbf_cb_desc* d)
 {
@@ -752,7 +1409,7 @@ extern "C" int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets,
     const int Fo = desc->nchans / desc->fscrunch;
     int ohs = desc->nsamples_per_heap_out ? desc->nsamples_per_heap_out : 8192 / Fo;
     FBF_REQUIRE(ohs > 0, FBF_ERR_ARG, "fbf_cb_plan_create: output heap holds no samples");
-    auto* p = new fbf_cb_plan{*desc, ohs, 0, 0, nullptr};
+    auto* p = new fbf_cb_plan{*desc, ohs, 0, 0, nullptr, nullptr, 0};
     hipError_t e = hipMalloc(&p->d_levels, sizeof(float) * 2 * Fo);
     if (e != hipSuccess) {
         delete p;
@@ -789,6 +1446,10 @@ extern "C" int fbf_cb_plan_destroy(fbf_cb_plan* plan)
 {
     if (!plan) return FBF_OK;
     hipError_t e = hipFree(plan->d_levels);
+    if (plan->staging) {
+        const hipError_t e2 = hipFree(plan->staging);
+        if (e == hipSuccess) e = e2;
+    }
     delete plan;
     if (e != hipSuccess) return hip_status(e, "hipFree(levels)");
     return FBF_OK;
@@ -800,23 +1461,32 @@ struct CbVariant {
     int mfma;        // 32: v_mfma_i32_32x32x32_i8, 16: v_mfma_i32_16x16x64_i8
     int rows;        // row tiles per wave (RT for 32x32, ROWT for 16x16)
     bool lds;        // voltage tile staged through LDS and shared by the work-group
+    bool tr;         // transposed tiles (cb_tr_k): voltages as the A operand
 };
 static const CbVariant kCbVariants[] = {
-    {0, 0, false},   // auto
-    {32, 4, false},  // 64 beams x 32 samples per wave
-    {32, 2, false},  // 32 beams x 32 samples per wave
-    {16, 8, false},  // 64 beams x 16 samples per wave
-    {16, 4, false},  // 32 beams x 16 samples per wave
-    {32, 1, true},   // LDS: 16 beams per wave, 64 per work-group
-    {32, 2, true},   // LDS: 32 beams per wave, 128 per work-group
-    {32, 4, true},   // LDS: 64 beams per wave, 256 per work-group
+    {0, 0, false, false},   // auto
+    {32, 4, false, false},  // 64 beams x 32 samples per wave
+    {32, 2, false, false},  // 32 beams x 32 samples per wave
+    {16, 8, false, false},  // 64 beams x 16 samples per wave
+    {16, 4, false, false},  // 32 beams x 16 samples per wave
+    {32, 1, true, false},   // LDS: 16 beams per wave, 64 per work-group
+    {32, 2, true, false},   // LDS: 32 beams per wave, 128 per work-group
+    {32, 4, true, false},   // LDS: 64 beams per wave, 256 per work-group
+    {32, 2, true, true},    // transposed: 32 beams per wave
+    {32, 4, true, true},    // transposed: 64 beams per wave
 };
+constexpr int kCbVariantLds = 5;
+constexpr int kCbVariantTr = 8;
 
-static CbVariant cb_pick_variant(const fbf_cb_desc& d, int variant)
+// The transposed kernel where it applies (int8 and power output), else the
+// LDS-staged one.
+static CbVariant cb_pick_variant(const fbf_cb_desc& d, int variant, int mode, long long T)
 {
-    if (variant > 0) return kCbVariants[variant];
-    (void)d;
-    return kCbVariants[5];
+    CbVariant v = kCbVariants[variant > 0 ? variant : kCbVariantTr];
+    if (v.tr && (mode != MODE_INT8 || T % 16 != 0 || 2 * T / 32 >= (1ll << 31) ||
+                 !cb_tr_supported(d.nantennas, d.tscrunch, d.fscrunch, v.rows)))
+        v = kCbVariants[kCbVariantLds];
+    return v;
 }
 
 extern "C" int fbf_cb_variant_count(int* count)
@@ -860,7 +1530,8 @@ extern "C" int fbf_cb_output_bytes(const fbf_cb_plan* plan, size_t ftpa_bytes, s
 // columns that are `vstride` bytes apart, the beamformer's antennas starting
 // `voff` bytes into each column.
 static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T, int vstride, int voff,
-                     const void* fba, size_t weights_bytes, void* out, size_t out_bytes, fbf_stream_t stream)
+                     const void* fba, size_t weights_bytes, void* out, size_t out_bytes, fbf_stream_t stream,
+                     void* workspace = nullptr, size_t workspace_bytes = 0)
 {
     FBF_REQUIRE(plan, FBF_ERR_ARG, "fbf_cb: null plan");
     FBF_REQUIRE(ftpa && fba && out, FBF_ERR_ARG, "fbf_cb: null buffer");
@@ -892,8 +1563,60 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
                     (reinterpret_cast<uintptr_t>(out) & 15) == 0,
                 FBF_ERR_ARG, "fbf_cb: buffers must be 16-byte aligned");
     if (T == 0) return FBF_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void* kout = out;
+    if (mode == MODE_INT8) {
+        if (workspace) {
+            FBF_REQUIRE(workspace_bytes >= expect && (reinterpret_cast<uintptr_t>(workspace) & 15) == 0,
+                        FBF_ERR_SIZE, "fbf_cb: workspace too small or not 16-byte aligned");
+            kout = workspace;
+        } else {
+            const int st = cb_staging(plan, expect, s, &kout);
+            if (st) return st;
+        }
+    }
 
-    const CbVariant var = cb_pick_variant(d, plan->variant);
+    const CbVariant var = cb_pick_variant(d, plan->variant, mode, T);
+    if (var.tr) {
+        CbTrArgs ta{};
+        CbArgs& a = ta.p;
+        a.ftpa = static_cast<const int8_t*>(ftpa);
+        a.fba = static_cast<const int8_t*>(fba);
+        a.out = kout;
+        a.offsets = plan->d_levels;
+        a.scales = plan->d_levels + Fo;
+        a.nsamples = T;
+        a.nsamples_out = To;
+        a.A = d.nantennas;
+        a.B = d.nbeams;
+        a.F = d.nchans;
+        a.Fo = Fo;
+        a.ts = d.tscrunch;
+        a.fs = d.fscrunch;
+        a.ohs = plan->ohs;
+        a.quant = d.quant_mode;
+        a.vstride = vstride;
+        a.voff = voff;
+        const int rt = var.rows;
+        const int W = 2 * d.tscrunch;
+        const int spt = W >= 32 ? 1 : 32 / W;
+        ta.tps = W >= 32 ? W / 32 : 1;
+        const int nslot = cb_tr_nslot(rt, W);
+        ta.ntp = (nslot / spt) * ta.tps;
+        ta.ntiles = (int)(2 * T / 32);
+        const int passes = (ta.ntiles + ta.ntp - 1) / ta.ntp;
+        const int beams_per_wg = 16 * rt * (4 / d.fscrunch);
+        ta.n_bgroups = (d.nbeams + beams_per_wg - 1) / beams_per_wg;
+        const int ksteps = (2 * d.nantennas + 31) / 32;
+        hipError_t e = rt == 2 ? launch_tr<2>(ta, d.fscrunch, W, ksteps, passes, s)
+                               : launch_tr<4>(ta, d.fscrunch, W, ksteps, passes, s);
+        if (e != hipSuccess) return hip_status(e, "cb_tr_k launch");
+        e = launch_tbtf(tbtf_args(static_cast<const int8_t*>(kout), static_cast<int8_t*>(out), To, d.nbeams, Fo,
+                                  plan->ohs),
+                        s);
+        if (e != hipSuccess) return hip_status(e, "tbtf_k launch");
+        return FBF_OK;
+    }
     const int kbytes = var.mfma == 32 ? 32 : 64;
     const int ksteps = (2 * d.nantennas + kbytes - 1) / kbytes;
     const int beams_per_wave = var.mfma == 32 ? 16 * var.rows : 8 * var.rows;
@@ -910,7 +1633,7 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
     CbArgs a;
     a.ftpa = static_cast<const int8_t*>(ftpa);
     a.fba = static_cast<const int8_t*>(fba);
-    a.out = out;
+    a.out = kout;
     a.offsets = plan->d_levels;
     a.scales = plan->d_levels + Fo;
     a.nsamples = T;
@@ -940,7 +1663,6 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
     }
     const long long nblocks = (long long)Fo * a.n_tgroups * a.n_bgroups;
     FBF_REQUIRE(nblocks > 0 && nblocks < (1ll << 31), FBF_ERR_UNSUPPORTED, "fbf_cb: grid too large");
-    hipStream_t s = static_cast<hipStream_t>(stream);
     dim3 grid((unsigned)nblocks);
     hipError_t e;
     if (var.lds) {
@@ -966,6 +1688,12 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
             e = launch_mode<MODE_POWER>(a, var.rows, ksteps, grid, s);
     }
     if (e != hipSuccess) return hip_status(e, "cb_mfma_k launch");
+    if (mode == MODE_INT8) {
+        e = launch_tbtf(tbtf_args(static_cast<const int8_t*>(kout), static_cast<int8_t*>(out), To, d.nbeams, Fo,
+                                  plan->ohs),
+                        s);
+        if (e != hipSuccess) return hip_status(e, "tbtf_k launch");
+    }
     return FBF_OK;
 }
 
@@ -1007,6 +1735,29 @@ extern "C" int fbf_cb_beamform(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_
                                size_t weights_bytes, void* tbtf, size_t tbtf_bytes, fbf_stream_t stream)
 {
     return cb_run(plan, MODE_INT8, ftpa, ftpa_bytes, fba_weights, weights_bytes, tbtf, tbtf_bytes, stream);
+}
+
+extern "C" int fbf_cb_beamform_ws(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes, const void* fba_weights,
+                                  size_t weights_bytes, void* tbtf, size_t tbtf_bytes, void* workspace,
+                                  size_t workspace_bytes, fbf_stream_t stream)
+{
+    FBF_REQUIRE(plan, FBF_ERR_ARG, "fbf_cb: null plan");
+    FBF_REQUIRE(workspace, FBF_ERR_ARG, "fbf_cb_beamform_ws: null workspace");
+    long long T = 0;
+    int st = cb_nsamples(plan, ftpa_bytes, &T);
+    if (st) return st;
+    return cb_launch(plan, MODE_INT8, ftpa, T, 2 * plan->d.nantennas, 0, fba_weights, weights_bytes, tbtf,
+                     tbtf_bytes, stream, workspace, workspace_bytes);
+}
+
+extern "C" int fbf_cb_plan_reserve(fbf_cb_plan* plan, long long nsamples)
+{
+    FBF_REQUIRE(plan && nsamples >= 0, FBF_ERR_ARG, "fbf_cb_plan_reserve: bad argument");
+    FBF_REQUIRE(nsamples % plan->d.tscrunch == 0, FBF_ERR_SIZE, "fbf_cb_plan_reserve: nsamples % tscrunch != 0");
+    const size_t bytes = (size_t)(nsamples / plan->d.tscrunch) * plan->d.nbeams * (plan->d.nchans / plan->d.fscrunch);
+    if (bytes == 0) return FBF_OK;
+    void* p = nullptr;
+    return cb_staging(plan, bytes, nullptr, &p);
 }
 
 extern "C" int fbf_cb_beam_sums(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes, const void* fba_weights,
@@ -1067,7 +1818,7 @@ extern "C" int fbf_cb_autotune(fbf_cb_plan* plan, long long nsamples, int reps, 
     float bms = 1e30f;
     for (int var = 1; var < nvar; ++var) {
         std::vector<int> chunks{0};
-        if (kCbVariants[var].lds)
+        if (kCbVariants[var].lds && !kCbVariants[var].tr)
             for (int jc = jc_max / 2; jc >= 1; jc >>= 1) chunks.push_back(jc);
         for (int jc : chunks) {
             plan->variant = var;

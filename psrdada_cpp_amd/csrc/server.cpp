@@ -181,11 +181,14 @@ int enqueue_block(fbf_server* s, Request* r)
     off += align_up(f.block_bytes);
     void* cb_dev = mem + off;
     off += align_up(f.cb_bytes);
+    void* cb_stage = mem + off;
+    off += align_up(f.cb_bytes);
     void* ib_dev = mem + off;
     int e = fbf_split_transpose(input, f.block_bytes, ftpa, f.block_bytes, f.d.nantennas, f.d.nantennas, 0,
                                 f.d.nchans, f.d.nsamples_per_heap, st);
     if (e) return e;
-    e = fbf_cb_beamform(f.cb, ftpa, f.block_bytes, f.weights[r->wbuf], f.weights_bytes, cb_dev, f.cb_bytes, st);
+    e = fbf_cb_beamform_ws(f.cb, ftpa, f.block_bytes, f.weights[r->wbuf], f.weights_bytes, cb_dev, f.cb_bytes,
+                           cb_stage, f.cb_bytes, st);
     if (e) return e;
     if (f.ib) {
         e = fbf_ib_beamform(f.ib, input, f.block_bytes, ib_dev, f.ib_bytes, st);
@@ -199,7 +202,7 @@ int enqueue_block(fbf_server* s, Request* r)
 
 size_t block_device_bytes(const Feed& f, bool on_device)
 {
-    return (on_device ? 0 : align_up(f.block_bytes)) + align_up(f.block_bytes) + align_up(f.cb_bytes) +
+    return (on_device ? 0 : align_up(f.block_bytes)) + align_up(f.block_bytes) + 2 * align_up(f.cb_bytes) +
            align_up(f.ib_bytes);
 }
 

@@ -103,6 +103,9 @@ def test_cb_power_exact(gpu, oracle, F, T, A, B, ts, fs):
     (8, 2048, 64, 512, 16, 1, 0),           # config 2 geometry, 8 channels, default heap (1024 samples)
     (8, 1024, 44, 64, 16, 4, 64),
     (4, 4096, 40, 768, 64, 1, 0), (4, 1024, 8, 64, 4, 1, 0),
+    # TBTF reorder edge tiles: channels not a multiple of 4, more than 64
+    # channels, partial beam tiles, heaps that are not a multiple of the tile
+    (6, 768, 8, 40, 2, 1, 24), (132, 64, 4, 8, 2, 1, 8), (10, 512, 12, 72, 4, 2, 0), (20, 1200, 16, 24, 3, 1, 50),
 ])
 def test_cb_int8_tbtf_bit_exact(gpu, oracle, mode, F, T, A, B, ts, fs, ohs):
     capi = gpu
@@ -288,11 +291,15 @@ def test_weights_fp64_phase_config4(gpu, oracle, epoch):
     assert (diff > 0).mean() < 1e-3
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9])
 @pytest.mark.parametrize("F,T,A,B,ts,fs", [(4, 512, 64, 512, 16, 1), (4, 256, 44, 96, 4, 2), (2, 512, 8, 64, 16, 1),
-                                            (2, 256, 40, 48, 64, 1), (2, 256, 128, 32, 8, 1)])
+                                            (2, 256, 40, 48, 64, 1), (2, 256, 128, 32, 8, 1),
+                                            (4, 512, 40, 96, 4, 4), (4, 1024, 64, 256, 64, 2), (2, 256, 16, 48, 8, 1),
+                                            (8, 256, 24, 72, 32, 4), (4, 512, 64, 136, 16, 2)])
 def test_cb_every_variant_bit_exact(gpu, oracle, variant, F, T, A, B, ts, fs):
-    """Every kernel variant (MFMA shape x tile) gives the oracle's sums and power."""
+    """Every kernel variant (MFMA shape x tile) gives the oracle's sums and power
+    (the transposed variants 8 and 9 serve int8 and power output and fall back
+    to the LDS variant for sums and for geometries they do not cover)."""
     capi = gpu
     ftpa = gaussian_int8(1300 + A + variant, F * T * 2 * A * 2, 16.0)
     w = unit_weights(1400 + B, F * B * A)
@@ -307,3 +314,35 @@ def test_cb_every_variant_bit_exact(gpu, oracle, variant, F, T, A, B, ts, fs):
     d_o = capi.DeviceBuffer((F // fs) * (T // ts) * B)
     plan.beamform(d_v, d_w, d_o)
     np.testing.assert_array_equal(d_o.download(), oracle.cb_tbtf(p, off, sc, T // ts))
+    d_p = capi.DeviceBuffer((F // fs) * (T // ts) * B * 8)
+    plan.beam_power(d_v, d_w, d_p)
+    np.testing.assert_array_equal(d_p.download(np.float64).reshape(F // fs, T // ts, B), p)
+
+
+@pytest.mark.parametrize("variant", [5, 8, 9])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_cb_quantiser_rounding_boundaries(gpu, oracle, variant, mode):
+    """Levels that put many outputs exactly on (or next to) rounding
+    boundaries: small integer voltages with unit weights make the power small
+    integers, and offset/scale of 0/2, 1/4, 0.5/1, 0/1 give exact
+    half-integers and integers; a zero and a negative scale cover the
+    non-finite and sign cases.  Every variant must match the oracle's
+    IEEE-division quantiser exactly."""
+    capi = gpu
+    F, T, A, B, ts = 8, 256, 8, 32, 4
+    rng = np.random.default_rng(77 + variant)
+    ftpa = rng.integers(-2, 3, F * T * 2 * A * 2).astype(np.int8)
+    w = np.zeros(F * B * A * 2, np.int8)
+    w[0::2] = 1
+    off = np.array([0.0, 1.0, 0.5, 0.0, 3.0, 0.0, 40.0, 0.25], np.float32)
+    sc = np.array([2.0, 4.0, 1.0, 1.0, 0.0, -2.0, 0.5, 1e-40], np.float32)
+    plan = capi.CbPlan(A, B, F, ts, 1, off, sc, nsamples_per_heap_out=T // ts, quant_mode=mode)
+    plan.set_variant(variant)
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    d_o = capi.DeviceBuffer(F * (T // ts) * B)
+    plan.beamform(d_v, d_w, d_o)
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, 1)
+    ref = oracle.cb_tbtf(p, off, sc, T // ts, mode)
+    np.testing.assert_array_equal(d_o.download(), ref)
+    # the levels really produce ties: half-integer quotients in channel 0
+    assert np.any(p[0] % 2 == 1)

@@ -122,7 +122,7 @@ def test_admission_respects_hbm_budget(gpu, oracle):
     capi = gpu
     A, B, F = FEEDS[0]
     block = NHG * A * F * 256 * 4
-    budget = 2 * block + (1 << 20) + 2 * 256 + 64 * 1024   # input + FTPA + outputs of one block
+    budget = 2 * block + (1 << 20) + 128 * 1024 + 2 * 256 + 64 * 1024   # input + FTPA + staging + outputs of one block
     srv = capi.Server(8, budget)
     cbl, ibl = _feed_levels(oracle, A, F)
     fid = srv.add_feed(A, B, F, NHG, TS, 1, cbl, ib=(16, 1), ib_levels=ibl, nsamples_per_heap_out=(NHG * 256) // TS)

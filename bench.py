@@ -186,13 +186,13 @@ def main():
         "kernel_hbm_gbs": {"split_transpose": round(st_bytes / (st_ms * 1e-3) / 1e9, 1),
                            "incoherent_beamformer": round(ib_bytes / (ib_ms * 1e-3) / 1e9, 1)},
         "roofline": {
-            "kernel": "cb_mfma_k (coherent beamformer)",
+            "kernel": "coherent beamformer (cb_tr_k int8 MFMA + detect/quantise, then tbtf_k reorder)",
             "bound": "mfma",
             "achieved": round(cb_tops, 2),
             "peak": wl.PEAK_INT8_TOPS,
             "unit": "TOP/s",
             "frac": round(cb_tops / wl.PEAK_INT8_TOPS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(),
             "ops_per_launch": ops,
         },
     }
@@ -210,6 +210,24 @@ def main():
         result["cpu_baseline"] = cpu_baseline(cfg, sub, taftp, args.cpu_channels)
     print(json.dumps(result), flush=True)
     return 0
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cb_64ant_64ch_512beams_hbm_traffic.txt")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the coherent-beam kernels (beamformer +
+    reorder) on this workload, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary committed under profiles/ (counter readings scaled by what they
+    read for a plain 1 GiB copy); None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def calibrate_copy():

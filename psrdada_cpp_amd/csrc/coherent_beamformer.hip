@@ -843,7 +843,8 @@ struct TrStage {
     static constexpr int M = (NI + NW - 1) / NW;           // per wave (the rest load into a dummy run)
     static constexpr int SET = NI * 1024;                  // bytes per set
     static constexpr int NB = 60 / NI < 2 ? 2 : (60 / NI > 8 ? 8 : 60 / NI);
-    static constexpr int WAIT = (NB - 2) * M;
+    static constexpr int G = NB >= 4 ? 2 : 1;             // steps per barrier
+    static constexpr int WAIT = (NB - 2 * G) * M;
     static constexpr int DUMMY = (NI % NW) ? NW * 1024 : 0;
 };
 
@@ -972,10 +973,10 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 
     const bool wave_active = beam_base < p.B;
 #pragma unroll
-    for (int q = 0; q < NB - 1; ++q) issue(q, q);
+    for (int q = 0; q < NB - St::G; ++q) issue(q, q);
     int step = 0;
-    int rs = 0;          // ring slot read this step
-    int is = NB - 1;     // ring slot filled this step
+    int rs = 0;              // ring slot read this step
+    int is = NB - St::G;     // next ring slot to fill
 
     const QuantConst qz = quant_const(p.offsets[fo], p.scales[fo]);
     for (int pass = 0; pass < ta.npasses; ++pass) {
@@ -990,13 +991,19 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 #pragma unroll
         for (int g = 0; g < NSLOT / SPT; ++g) {
             for (int x = 0; x < ta.tps; ++x) {
-                // This wave's loads of the current set are done once WAIT
-                // later loads remain; the barrier makes every wave's loads
-                // visible and frees the slot read in the previous step.
-                wait_vm_lgkm0<St::WAIT>();
-                block_barrier();
-                issue(step + NB - 1, is);
-                is = is + 1 == NB ? 0 : is + 1;
+                // Every G steps: this wave's loads of the next G sets are
+                // done once WAIT later loads remain; the barrier makes every
+                // wave's loads visible and frees the G slots read since the
+                // last barrier, which are refilled NB - G steps ahead.
+                if (step % St::G == 0) {
+                    wait_vm_lgkm0<St::WAIT>();
+                    block_barrier();
+#pragma unroll
+                    for (int j = 0; j < St::G; ++j) {
+                        issue(step + NB - St::G + j, is);
+                        is = is + 1 == NB ? 0 : is + 1;
+                    }
+                }
                 if (wave_active) {
                     const int8_t* src = lds + rs * St::SET;
                     v4i a[KSTEPS];
@@ -1215,15 +1222,16 @@ hipError_t launch_tr(const CbTrArgs& a, int fs, int W, int ksteps, int passes, h
 // ---------------------------------------------------------------------------
 // Staged [Fo][To][B] int8 -> TBTF [To/ohs][B][ohs][Fo]
 //
-// A work-group moves a tile of up to 64 beams x tt samples (within one output
-// heap) x up to 64 channels.  Loads are 8-byte runs of beams; the tile is
-// scattered into LDS as [t][b][f] (channel bytes contiguous) and written out
-// as 4-byte words of channels, so one wave stores whole rows of
+// A work-group moves a tile of up to 128 beams x tt samples (within one
+// output heap) x up to 64 channels.  Loads are 8-byte runs of beams (a tile
+// row is a whole 128-byte line, so no line is fetched by two work-groups);
+// the tile is scattered into LDS as [t][b][f] (channel bytes contiguous) and
+// written out as 4-byte words of channels, so one wave stores whole rows of
 // tt * Fo contiguous bytes per beam.  LDS dword layout: b pitch BP (odd),
-// t pitch TP = 64 * BP + 16, so that the word reads of a wave (16 words of
+// t pitch TP = 128 * BP + 16, so that the word reads of a wave (16 words of
 // one (t, b) row x 4 samples) fall in distinct banks.
 // ---------------------------------------------------------------------------
-constexpr int kTrBeams = 64;
+constexpr int kTrBeams = 128;
 constexpr int kTrMaxChans = 64;
 constexpr int kTrLdsBudget = 40 * 1024;
 
@@ -1476,16 +1484,27 @@ static const CbVariant kCbVariants[] = {
     {32, 4, true, true},    // transposed: 64 beams per wave
 };
 constexpr int kCbVariantLds = 5;
-constexpr int kCbVariantTr = 8;
+constexpr int kCbVariantTr2 = 8;
+constexpr int kCbVariantTr4 = 9;
 
-// The transposed kernel where it applies (int8 and power output), else the
-// LDS-staged one.
+static bool cb_tr_applies(const fbf_cb_desc& d, int rt, int mode, long long T)
+{
+    return mode == MODE_INT8 && T % 16 == 0 && 2 * T / 32 < (1ll << 31) &&
+           cb_tr_supported(d.nantennas, d.tscrunch, d.fscrunch, rt);
+}
+
+// Automatic choice: the transposed kernel with 64 beams per wave, else with
+// 32 beams per wave, else the LDS-staged kernel (which also serves int32
+// sums, power output and the geometries the transposed kernel does not).
 static CbVariant cb_pick_variant(const fbf_cb_desc& d, int variant, int mode, long long T)
 {
-    CbVariant v = kCbVariants[variant > 0 ? variant : kCbVariantTr];
-    if (v.tr && (mode != MODE_INT8 || T % 16 != 0 || 2 * T / 32 >= (1ll << 31) ||
-                 !cb_tr_supported(d.nantennas, d.tscrunch, d.fscrunch, v.rows)))
-        v = kCbVariants[kCbVariantLds];
+    if (variant <= 0) {
+        if (cb_tr_applies(d, 4, mode, T)) return kCbVariants[kCbVariantTr4];
+        if (cb_tr_applies(d, 2, mode, T)) return kCbVariants[kCbVariantTr2];
+        return kCbVariants[kCbVariantLds];
+    }
+    const CbVariant v = kCbVariants[variant];
+    if (v.tr && !cb_tr_applies(d, v.rows, mode, T)) return kCbVariants[kCbVariantLds];
     return v;
 }
 

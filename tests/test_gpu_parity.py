@@ -285,8 +285,10 @@ def test_weights_fp64_phase_config4(gpu, oracle, epoch):
     w, ph = _weights_gpu(capi, d, fr, epoch, A, B, F)
     ref, frac = oracle.weights_f64(d, fr, epoch, A, B, F, want_phase=True)
     err_rad = np.abs(ph.astype(np.float64) - frac).max() * 2 * np.pi
-    assert err_rad <= 2 * np.pi * 2.0 ** -25
     diff = np.abs(w.astype(int) - ref.astype(int))
+    print(f"weights 64x1024x64 epoch {epoch}: max phase error {err_rad:.3e} rad, "
+          f"components 1 LSB off the FP64 oracle {(diff > 0).mean():.3e}, max diff {diff.max()}")
+    assert err_rad <= 2 * np.pi * 2.0 ** -25
     assert diff.max() <= 1
     assert (diff > 0).mean() < 1e-3
 

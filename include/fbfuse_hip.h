@@ -207,6 +207,16 @@ int fbf_ib_beamform(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes,
 int fbf_ib_power(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes,
                  double* power, size_t power_bytes, fbf_stream_t stream);
 
+/* Split-transpose of a full-antenna block (all antennas, no subset) and the
+ * incoherent beam of the same block in one pass over the input: the FTPA
+ * output equals fbf_split_transpose's and the TF output equals
+ * fbf_ib_beamform's with ib_plan.  No counterpart in the reference (there the
+ * two operators read the block separately); needs the plan's fscrunch to be
+ * 1 and its tscrunch to divide 32, else FBF_ERR_UNSUPPORTED. */
+int fbf_split_transpose_ib(const void* taftp, size_t taftp_bytes, void* ftpa, size_t ftpa_bytes,
+                           int nchans, int nsamples_per_heap, fbf_ib_plan* ib_plan, void* tf,
+                           size_t tf_bytes, fbf_stream_t stream);
+
 /* ---- delay -> weights (WeightsManager.cu:14-123) ----------------------- */
 typedef struct fbf_weights_plan fbf_weights_plan;
 

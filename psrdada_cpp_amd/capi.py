@@ -122,6 +122,7 @@ _SIGS = {
     "fbf_batch_autotune": [vp, i32],
     "fbf_batch_run": [vp, vp, sz, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), i32, vp],
     "fbf_cb_beamform_ws": [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp],
+    "fbf_split_transpose_ib": [vp, sz, vp, sz, i32, i32, vp, vp, sz, vp],
     "fbf_cb_plan_reserve": [vp, C.c_longlong],
     "fbf_server_create": [i32, sz, pvp],
     "fbf_server_destroy": [vp],
@@ -345,6 +346,12 @@ def split_transpose(taftp, ftpa, total_nantennas, used_nantennas, start_antenna,
                     nsamples_per_heap=256, stream=None):
     call("fbf_split_transpose", taftp.ptr, taftp.nbytes, ftpa.ptr, ftpa.nbytes, total_nantennas,
          used_nantennas, start_antenna, nchans, nsamples_per_heap, _s(stream))
+
+
+def split_transpose_ib(taftp, ftpa, nchans, ib_plan, tf, nsamples_per_heap=256, stream=None):
+    """Full-antenna split-transpose and incoherent beam in one pass."""
+    call("fbf_split_transpose_ib", taftp.ptr, taftp.nbytes, ftpa.ptr, ftpa.nbytes, nchans, nsamples_per_heap,
+         ib_plan.handle, tf.ptr, tf.nbytes, _s(stream))
 
 
 class CbPlan:

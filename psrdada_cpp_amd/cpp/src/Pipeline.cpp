@@ -136,14 +136,29 @@ void Pipeline::process(VoltageVectorType const& taftp, PowerVectorType& tbtf, Po
                                          _weights.data(), _weights.bytes(), nullptr, _processing_stream));
     _split_transpose_output.resize(_nsamples_per_dada_block * _config.npol() * _config.nchans() *
                                    _config.cb_nantennas());
-    FBF_ERROR_CHECK(fbf_split_transpose(taftp.data(), taftp.bytes(), _split_transpose_output.data(),
-                                        _split_transpose_output.bytes(), (int)_config.total_nantennas(),
-                                        (int)_config.cb_nantennas(), (int)_config.cb_antenna_offset(),
-                                        (int)_config.nchans(), (int)_config.nsamples_per_heap(), _processing_stream));
+    // A whole-block split can produce the incoherent beam in the same pass
+    // over the input when the IB scrunch allows it.
+    const bool fused = _config.cb_nantennas() == _config.total_nantennas() && _config.cb_antenna_offset() == 0 &&
+                       _config.ib_fscrunch() == 1 && 32 % _config.ib_tscrunch() == 0 &&
+                       _config.nsamples_per_heap() % 32 == 0;
+    if (fused) {
+        FBF_ERROR_CHECK(fbf_split_transpose_ib(taftp.data(), taftp.bytes(), _split_transpose_output.data(),
+                                               _split_transpose_output.bytes(), (int)_config.nchans(),
+                                               (int)_config.nsamples_per_heap(), _ib_plan, tf.data(), tf.bytes(),
+                                               _processing_stream));
+    } else {
+        FBF_ERROR_CHECK(fbf_split_transpose(taftp.data(), taftp.bytes(), _split_transpose_output.data(),
+                                            _split_transpose_output.bytes(), (int)_config.total_nantennas(),
+                                            (int)_config.cb_nantennas(), (int)_config.cb_antenna_offset(),
+                                            (int)_config.nchans(), (int)_config.nsamples_per_heap(),
+                                            _processing_stream));
+    }
     FBF_ERROR_CHECK(fbf_cb_beamform(_cb_plan, _split_transpose_output.data(), _split_transpose_output.bytes(),
                                     _weights.data(), _weights.bytes(), tbtf.data(), tbtf.bytes(),
                                     _processing_stream));
-    FBF_ERROR_CHECK(fbf_ib_beamform(_ib_plan, taftp.data(), taftp.bytes(), tf.data(), tf.bytes(), _processing_stream));
+    if (!fused)
+        FBF_ERROR_CHECK(
+            fbf_ib_beamform(_ib_plan, taftp.data(), taftp.bytes(), tf.data(), tf.bytes(), _processing_stream));
 }
 
 bool Pipeline::operator()(RawBytes& data)

@@ -19,6 +19,7 @@ struct fbf_batch {
     std::vector<fbf_subarray_desc> subs;
     std::vector<fbf_cb_plan*> plans;
     fbf_ib_plan* ib = nullptr;
+    bool ib_fused = false;         // incoherent beam computed by the split-transpose pass
     void* ftpa = nullptr;          // full-antenna FTPA workspace
     size_t ftpa_bytes = 0;
     std::vector<hipStream_t> streams;
@@ -130,6 +131,7 @@ extern "C" int fbf_batch_enable_ib(fbf_batch* b, int tscrunch, int fscrunch, int
     if (st) return st;
     if (b->ib) fbf_ib_plan_destroy(b->ib);
     b->ib = p;
+    b->ib_fused = fscrunch == 1 && tscrunch > 0 && 32 % tscrunch == 0 && b->S % 32 == 0;
     b->graph_ptrs.clear();   // forces a new capture
     return FBF_OK;
 }
@@ -181,19 +183,33 @@ static int batch_enqueue(fbf_batch* b, const void* taftp, size_t taftp_bytes, co
                          void* const* outputs, hipStream_t stream)
 {
     const int n = (int)b->subs.size();
-    // incoherent beam on its own branch: it only reads the TAFTP block
+    // The incoherent beam comes out of the split-transpose pass when its
+    // scrunch allows (fbf_split_transpose_ib), else it runs on its own
+    // branch (it only reads the TAFTP block).
+    bool ib_branch = false;
+    int st = FBF_OK;
     if (b->ib) {
-        FBF_HIP_TRY(hipEventRecord(b->fork_ib, stream));
-        hipStream_t s = b->streams[n];
-        FBF_HIP_TRY(hipStreamWaitEvent(s, b->fork_ib, 0));
         size_t ob = 0;
-        int st = fbf_ib_output_bytes(b->ib, taftp_bytes, &ob);
-        if (st == FBF_OK) st = fbf_ib_beamform(b->ib, taftp, taftp_bytes, outputs[n], ob, s);
+        st = fbf_ib_output_bytes(b->ib, taftp_bytes, &ob);
         if (st) return st;
-        FBF_HIP_TRY(hipEventRecord(b->join[n], s));
+        if (b->ib_fused) {
+            st = fbf_split_transpose_ib(taftp, taftp_bytes, b->ftpa, b->ftpa_bytes, b->F, b->S, b->ib, outputs[n],
+                                        ob, stream);
+            if (st) return st;
+        } else {
+            ib_branch = true;
+            FBF_HIP_TRY(hipEventRecord(b->fork_ib, stream));
+            hipStream_t s = b->streams[n];
+            FBF_HIP_TRY(hipStreamWaitEvent(s, b->fork_ib, 0));
+            st = fbf_ib_beamform(b->ib, taftp, taftp_bytes, outputs[n], ob, s);
+            if (st) return st;
+            FBF_HIP_TRY(hipEventRecord(b->join[n], s));
+        }
     }
-    int st = fbf_split_transpose(taftp, taftp_bytes, b->ftpa, b->ftpa_bytes, b->A, b->A, 0, b->F, b->S, stream);
-    if (st) return st;
+    if (!b->ib || !b->ib_fused) {
+        st = fbf_split_transpose(taftp, taftp_bytes, b->ftpa, b->ftpa_bytes, b->A, b->A, 0, b->F, b->S, stream);
+        if (st) return st;
+    }
     FBF_HIP_TRY(hipEventRecord(b->fork_cb, stream));
     for (int i = 0; i < n; ++i) {
         hipStream_t s = b->streams[i];
@@ -205,7 +221,7 @@ static int batch_enqueue(fbf_batch* b, const void* taftp, size_t taftp_bytes, co
         if (st) return st;
         FBF_HIP_TRY(hipEventRecord(b->join[i], s));
     }
-    for (int i = 0; i < n + (b->ib ? 1 : 0); ++i) FBF_HIP_TRY(hipStreamWaitEvent(stream, b->join[i], 0));
+    for (int i = 0; i < n + (ib_branch ? 1 : 0); ++i) FBF_HIP_TRY(hipStreamWaitEvent(stream, b->join[i], 0));
     return FBF_OK;
 }
 

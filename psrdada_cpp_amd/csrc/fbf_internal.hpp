@@ -40,6 +40,15 @@ int hip_status(hipError_t e, const char* what);
 
 namespace fbf {
 
+// What the fused split-transpose + incoherent-beam path needs from an
+// incoherent-beam plan (incoherent_beamformer.hip).
+struct IbFuseParams {
+    const float* offsets;   // device, [nchans / fscrunch]
+    const float* scales;
+    int nantennas, nchans, nsamples_per_heap, tscrunch, fscrunch, quant;
+};
+int ib_fuse_params(const fbf_ib_plan* plan, IbFuseParams* out);
+
 // Number of XCDs on MI355X; work-groups are dispatched round robin over them.
 constexpr int kNumXcd = 8;
 

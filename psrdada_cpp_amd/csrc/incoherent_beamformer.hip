@@ -232,6 +232,21 @@ static int ib_run(fbf_ib_plan* plan, int mode, const void* taftp, size_t taftp_b
     return FBF_OK;
 }
 
+int fbf::ib_fuse_params(const fbf_ib_plan* plan, IbFuseParams* out)
+{
+    FBF_REQUIRE(plan && out, FBF_ERR_ARG, "fbf_ib: null plan");
+    const int fo = plan->d.nchans / plan->d.fscrunch;
+    out->offsets = plan->d_levels;
+    out->scales = plan->d_levels + fo;
+    out->nantennas = plan->d.nantennas;
+    out->nchans = plan->d.nchans;
+    out->nsamples_per_heap = plan->d.nsamples_per_heap;
+    out->tscrunch = plan->d.tscrunch;
+    out->fscrunch = plan->d.fscrunch;
+    out->quant = plan->d.quant_mode;
+    return FBF_OK;
+}
+
 extern "C" int fbf_ib_beamform(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes, void* tf, size_t tf_bytes,
                                fbf_stream_t stream)
 {

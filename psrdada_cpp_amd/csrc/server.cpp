@@ -184,15 +184,20 @@ int enqueue_block(fbf_server* s, Request* r)
     void* cb_stage = mem + off;
     off += align_up(f.cb_bytes);
     void* ib_dev = mem + off;
-    int e = fbf_split_transpose(input, f.block_bytes, ftpa, f.block_bytes, f.d.nantennas, f.d.nantennas, 0,
-                                f.d.nchans, f.d.nsamples_per_heap, st);
+    const bool fused = f.ib && f.d.ib_fscrunch == 1 && 32 % f.d.ib_tscrunch == 0;
+    int e = fused ? fbf_split_transpose_ib(input, f.block_bytes, ftpa, f.block_bytes, f.d.nchans,
+                                           f.d.nsamples_per_heap, f.ib, ib_dev, f.ib_bytes, st)
+                  : fbf_split_transpose(input, f.block_bytes, ftpa, f.block_bytes, f.d.nantennas, f.d.nantennas, 0,
+                                        f.d.nchans, f.d.nsamples_per_heap, st);
     if (e) return e;
     e = fbf_cb_beamform_ws(f.cb, ftpa, f.block_bytes, f.weights[r->wbuf], f.weights_bytes, cb_dev, f.cb_bytes,
                            cb_stage, f.cb_bytes, st);
     if (e) return e;
     if (f.ib) {
-        e = fbf_ib_beamform(f.ib, input, f.block_bytes, ib_dev, f.ib_bytes, st);
-        if (e) return e;
+        if (!fused) {
+            e = fbf_ib_beamform(f.ib, input, f.block_bytes, ib_dev, f.ib_bytes, st);
+            if (e) return e;
+        }
         if (r->ib_out) FBF_HIP_TRY(hipMemcpyAsync(r->ib_out, ib_dev, f.ib_bytes, hipMemcpyDeviceToHost, st));
     }
     FBF_HIP_TRY(hipMemcpyAsync(r->cb_out, cb_dev, f.cb_bytes, hipMemcpyDeviceToHost, st));

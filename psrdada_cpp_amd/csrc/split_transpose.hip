@@ -25,13 +25,24 @@ template <> struct VecOf<4> { using T = int2; };
 template <> struct VecOf<2> { using T = int; };
 template <> struct VecOf<1> { using T = int16_t; };
 
-template <int U>
+// Incoherent beam computed from the same tiles (all antennas of the block):
+// output [sample_out][channel] int8, tscrunch dividing the tile's 32 samples,
+// fscrunch 1 (IncoherentBeamformer.cu:10-69).
+struct IbTile {
+    int8_t* out;            // null: no incoherent beam
+    const float* offsets;
+    const float* scales;
+    int tscrunch, quant;
+};
+
+template <int U, bool IB>
 __global__ __launch_bounds__(256) void split_transpose_k(
     const int16_t* __restrict__ in, int16_t* __restrict__ out,
     int nheap_groups, int total_nantennas, int used_nantennas, int start_antenna,
-    int nchans, int sp_per_heap, int n_sp_tiles)
+    int nchans, int sp_per_heap, int n_sp_tiles, IbTile ib)
 {
     __shared__ int16_t tile[kTileA][kTileSP + kPad];
+    __shared__ int ibsum[4][kTileSP / 2];
     const int tid = threadIdx.x;
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
     const int spt = logical % n_sp_tiles;
@@ -41,6 +52,11 @@ __global__ __launch_bounds__(256) void split_transpose_k(
     if (f >= nchans) return;
 
     const size_t out_block = ((size_t)f * nheap_groups + hg) * sp_per_heap + (size_t)spt * kTileSP;
+    // Incoherent beam: a 16-byte row load holds 4 samples x 2 pols of one
+    // antenna, and one sdot4 of a 4-byte word with itself is the power of
+    // both pols of a sample.  This thread sums its 4 samples over the rows it
+    // loads (exact in int32: 4 * 128^2 per word, at most 128 rows).
+    int ibacc[4] = {0, 0, 0, 0};
 
     for (int ab = 0; ab < used_nantennas; ab += kTileA) {
         const int na = min(kTileA, used_nantennas - ab);
@@ -52,6 +68,12 @@ __global__ __launch_bounds__(256) void split_transpose_k(
             if (r < na) {
                 const size_t row = ((size_t)hg * total_nantennas + start_antenna + ab + r) * nchans + f;
                 const int4 v = *reinterpret_cast<const int4*>(in + row * sp_per_heap + (size_t)spt * kTileSP + col);
+                if (IB) {
+                    ibacc[0] = __builtin_amdgcn_sdot4(v.x, v.x, ibacc[0], false);
+                    ibacc[1] = __builtin_amdgcn_sdot4(v.y, v.y, ibacc[1], false);
+                    ibacc[2] = __builtin_amdgcn_sdot4(v.z, v.z, ibacc[2], false);
+                    ibacc[3] = __builtin_amdgcn_sdot4(v.w, v.w, ibacc[3], false);
+                }
                 uint32_t* dst = reinterpret_cast<uint32_t*>(&tile[r][col]);
                 dst[0] = (uint32_t)v.x;
                 dst[1] = (uint32_t)v.y;
@@ -75,12 +97,80 @@ __global__ __launch_bounds__(256) void split_transpose_k(
         }
         __syncthreads();
     }
+    if (IB) {
+        // Lanes with the same tid % 8 hold the same 4 samples for different
+        // rows: sum over the 8 of them in the wave, then over the 4 waves.
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            ibacc[q] += __shfl_xor(ibacc[q], 8);
+            ibacc[q] += __shfl_xor(ibacc[q], 16);
+            ibacc[q] += __shfl_xor(ibacc[q], 32);
+        }
+        const int lane = tid & 63;
+        if (lane < 8) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ibsum[tid >> 6][lane * 4 + q] = ibacc[q];
+        }
+        __syncthreads();
+        // 32 samples per tile -> 32 / tscrunch outputs.
+        const int nout = (kTileSP / 2) / ib.tscrunch;
+        if (tid < nout) {
+            long long P = 0;
+            for (int c = tid * ib.tscrunch; c < (tid + 1) * ib.tscrunch; ++c)
+                P += (long long)ibsum[0][c] + ibsum[1][c] + ibsum[2][c] + ibsum[3][c];
+            const long long t0 = ((long long)hg * sp_per_heap + (long long)spt * kTileSP) / 2;
+            const long long to = t0 / ib.tscrunch + tid;
+            ib.out[to * nchans + f] = quantise((double)P, ib.offsets[f], ib.scales[f], ib.quant);
+        }
+    }
 }
 
 }  // namespace
 }  // namespace fbf
 
 using namespace fbf;
+
+namespace {
+
+template <bool IB>
+int st_launch_t(const void* taftp, void* ftpa, int total_nantennas, int used_nantennas, int start_antenna,
+                int nchans, int nsamples_per_heap, int nhg, IbTile ib, fbf_stream_t stream)
+{
+    const int sp_per_heap = 2 * nsamples_per_heap;
+    const int n_sp_tiles = sp_per_heap / kTileSP;
+    const size_t nblocks = (size_t)nchans * nhg * n_sp_tiles;
+    FBF_REQUIRE(nblocks < (1ull << 31), FBF_ERR_UNSUPPORTED, "fbf_split_transpose: grid too large");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const auto* in = static_cast<const int16_t*>(taftp);
+    auto* out = static_cast<int16_t*>(ftpa);
+    dim3 grid((unsigned)nblocks), block(256);
+    if (used_nantennas % 8 == 0)
+        split_transpose_k<8, IB><<<grid, block, 0, s>>>(in, out, nhg, total_nantennas, used_nantennas,
+                                                         start_antenna, nchans, sp_per_heap, n_sp_tiles, ib);
+    else if (used_nantennas % 4 == 0)
+        split_transpose_k<4, IB><<<grid, block, 0, s>>>(in, out, nhg, total_nantennas, used_nantennas,
+                                                         start_antenna, nchans, sp_per_heap, n_sp_tiles, ib);
+    else if (used_nantennas % 2 == 0)
+        split_transpose_k<2, IB><<<grid, block, 0, s>>>(in, out, nhg, total_nantennas, used_nantennas,
+                                                         start_antenna, nchans, sp_per_heap, n_sp_tiles, ib);
+    else
+        split_transpose_k<1, IB><<<grid, block, 0, s>>>(in, out, nhg, total_nantennas, used_nantennas,
+                                                         start_antenna, nchans, sp_per_heap, n_sp_tiles, ib);
+    FBF_LAUNCH_CHECK("split_transpose_k");
+    return FBF_OK;
+}
+
+int st_launch(const void* taftp, void* ftpa, int total_nantennas, int used_nantennas, int start_antenna, int nchans,
+              int nsamples_per_heap, int nhg, IbTile ib, fbf_stream_t stream)
+{
+    if (ib.out)
+        return st_launch_t<true>(taftp, ftpa, total_nantennas, used_nantennas, start_antenna, nchans,
+                                 nsamples_per_heap, nhg, ib, stream);
+    return st_launch_t<false>(taftp, ftpa, total_nantennas, used_nantennas, start_antenna, nchans,
+                              nsamples_per_heap, nhg, ib, stream);
+}
+
+}  // namespace
 
 extern "C" int fbf_split_transpose(const void* taftp, size_t taftp_bytes, void* ftpa,
                                    size_t ftpa_bytes, int total_nantennas, int used_nantennas,
@@ -109,26 +199,37 @@ extern "C" int fbf_split_transpose(const void* taftp, size_t taftp_bytes, void* 
     if (nhg == 0) return FBF_OK;
     FBF_REQUIRE(nhg < (1u << 20), FBF_ERR_UNSUPPORTED, "fbf_split_transpose: too many heap groups");
 
-    const int sp_per_heap = 2 * nsamples_per_heap;
-    const int n_sp_tiles = sp_per_heap / kTileSP;
-    const size_t nblocks = (size_t)nchans * nhg * n_sp_tiles;
-    FBF_REQUIRE(nblocks < (1ull << 31), FBF_ERR_UNSUPPORTED, "fbf_split_transpose: grid too large");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const auto* in = static_cast<const int16_t*>(taftp);
-    auto* out = static_cast<int16_t*>(ftpa);
-    dim3 grid((unsigned)nblocks), block(256);
-    if (used_nantennas % 8 == 0)
-        split_transpose_k<8><<<grid, block, 0, s>>>(in, out, (int)nhg, total_nantennas, used_nantennas,
-                                                     start_antenna, nchans, sp_per_heap, n_sp_tiles);
-    else if (used_nantennas % 4 == 0)
-        split_transpose_k<4><<<grid, block, 0, s>>>(in, out, (int)nhg, total_nantennas, used_nantennas,
-                                                     start_antenna, nchans, sp_per_heap, n_sp_tiles);
-    else if (used_nantennas % 2 == 0)
-        split_transpose_k<2><<<grid, block, 0, s>>>(in, out, (int)nhg, total_nantennas, used_nantennas,
-                                                     start_antenna, nchans, sp_per_heap, n_sp_tiles);
-    else
-        split_transpose_k<1><<<grid, block, 0, s>>>(in, out, (int)nhg, total_nantennas, used_nantennas,
-                                                     start_antenna, nchans, sp_per_heap, n_sp_tiles);
-    FBF_LAUNCH_CHECK("split_transpose_k");
-    return FBF_OK;
+    return st_launch(taftp, ftpa, total_nantennas, used_nantennas, start_antenna, nchans, nsamples_per_heap,
+                     (int)nhg, IbTile{nullptr, nullptr, nullptr, 1, 0}, stream);
+}
+
+extern "C" int fbf_split_transpose_ib(const void* taftp, size_t taftp_bytes, void* ftpa, size_t ftpa_bytes,
+                                      int nchans, int nsamples_per_heap, fbf_ib_plan* ib_plan, void* tf,
+                                      size_t tf_bytes, fbf_stream_t stream)
+{
+    FBF_REQUIRE(taftp && ftpa && ib_plan && tf, FBF_ERR_ARG, "fbf_split_transpose_ib: null argument");
+    IbFuseParams ibp;
+    int st = ib_fuse_params(ib_plan, &ibp);
+    if (st) return st;
+    FBF_REQUIRE(ibp.nchans == nchans && ibp.nsamples_per_heap == nsamples_per_heap, FBF_ERR_ARG,
+                "fbf_split_transpose_ib: incoherent-beam plan geometry differs from the block's");
+    FBF_REQUIRE(ibp.fscrunch == 1 && ibp.tscrunch > 0 && (kTileSP / 2) % ibp.tscrunch == 0, FBF_ERR_UNSUPPORTED,
+                "fbf_split_transpose_ib: needs fscrunch 1 and a tscrunch dividing 32");
+    const int A = ibp.nantennas;
+    FBF_REQUIRE(nsamples_per_heap > 0 && nsamples_per_heap % 32 == 0, FBF_ERR_UNSUPPORTED,
+                "fbf_split_transpose_ib: nsamples_per_heap must be a multiple of 32");
+    const size_t hg_bytes = (size_t)4 * A * nchans * nsamples_per_heap;
+    FBF_REQUIRE(taftp_bytes % hg_bytes == 0 && ftpa_bytes == taftp_bytes, FBF_ERR_SIZE,
+                "fbf_split_transpose_ib: input and output must be the same whole number of heap groups");
+    const size_t nhg = taftp_bytes / hg_bytes;
+    const size_t expect_tf = nhg * nsamples_per_heap / ibp.tscrunch * nchans;
+    FBF_REQUIRE(tf_bytes == expect_tf, FBF_ERR_SIZE,
+                "fbf_split_transpose_ib: incoherent-beam output size " + std::to_string(tf_bytes) +
+                    " != expected " + std::to_string(expect_tf));
+    FBF_REQUIRE((reinterpret_cast<uintptr_t>(taftp) & 15) == 0 && (reinterpret_cast<uintptr_t>(ftpa) & 15) == 0,
+                FBF_ERR_ARG, "fbf_split_transpose_ib: buffers must be 16-byte aligned");
+    if (nhg == 0) return FBF_OK;
+    FBF_REQUIRE(nhg < (1u << 20), FBF_ERR_UNSUPPORTED, "fbf_split_transpose_ib: too many heap groups");
+    return st_launch(taftp, ftpa, A, A, 0, nchans, nsamples_per_heap, (int)nhg,
+                     IbTile{static_cast<int8_t*>(tf), ibp.offsets, ibp.scales, ibp.tscrunch, ibp.quant}, stream);
 }

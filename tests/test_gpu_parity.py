@@ -223,6 +223,38 @@ def test_ib_bit_exact(gpu, oracle, A, F, nhg, ts, fs):
         np.testing.assert_array_equal(d_o.download(), oracle.ib_tf(p_ref, off, sc, mode))
 
 
+@pytest.mark.parametrize("A,F,nhg,ts", [(64, 64, 4, 16), (40, 8, 2, 32), (8, 16, 3, 1), (24, 4, 1, 4),
+                                         (128, 4, 2, 16), (12, 8, 2, 8)])
+def test_split_transpose_ib_fused(gpu, oracle, A, F, nhg, ts):
+    """One pass over a full-antenna block gives the oracle's FTPA and TF."""
+    capi = gpu
+    taftp = gaussian_int8(1200 + A + ts, nhg * A * F * 256 * 4, 16.0)
+    _, _, ibo, ibs = oracle.power_levels(16.0, 24.0, A, 16, 1, A, ts, 1)
+    off, sc = np.full(F, ibo, np.float32), np.full(F, ibs, np.float32)
+    d_in = _dev(capi, taftp)
+    d_ftpa = capi.DeviceBuffer(taftp.nbytes)
+    p_ref = oracle.ib_power(taftp, F, ts, 1, nhg, A)
+    for mode in (0, 1):
+        plan = capi.IbPlan(A, F, ts, 1, off, sc, quant_mode=mode)
+        d_tf = capi.DeviceBuffer(plan.output_bytes(taftp.nbytes))
+        d_ftpa.fill(0x5A)
+        capi.split_transpose_ib(d_in, d_ftpa, F, plan, d_tf)
+        np.testing.assert_array_equal(d_ftpa.download(), oracle.split_transpose(taftp, nhg, A, A, 0, F))
+        np.testing.assert_array_equal(d_tf.download(), oracle.ib_tf(p_ref, off, sc, mode))
+
+
+def test_split_transpose_ib_rejects_fscrunch(gpu):
+    capi = gpu
+    A, F, nhg = 8, 8, 1
+    plan = capi.IbPlan(A, F, 16, 2, np.zeros(F // 2, np.float32), np.ones(F // 2, np.float32))
+    d_in = capi.DeviceBuffer(nhg * A * F * 256 * 4)
+    d_ftpa = capi.DeviceBuffer(nhg * A * F * 256 * 4)
+    d_tf = capi.DeviceBuffer(plan.output_bytes(nhg * A * F * 256 * 4))
+    with pytest.raises(capi.FbfError) as ei:
+        capi.split_transpose_ib(d_in, d_ftpa, F, plan, d_tf)
+    assert ei.value.code == capi.FBF_ERR_UNSUPPORTED
+
+
 def test_ib_matches_reference_f32_within_tolerance(gpu, oracle):
     """IncoherentBeamformerTester.cu:124-145 geometry: 64 antennas, 64 channels,
     ts 16, fs 2, 32 heap groups, N(0, 32) voltages, levels 32/32, truncation."""

@@ -1258,6 +1258,34 @@ __global__ __launch_bounds__(256) void tbtf_k(TbtfArgs a)
     const int bv = min(kTrBeams, a.B - b0), fv = min(a.ft, a.Fo - f0), tv = min(a.tt, a.ohs - t0);
     const long long tg0 = heap * a.ohs + t0;
 
+    if ((fv & 3) == 0 && (f0 & 3) == 0) {
+        // Load 4 channels x 4 beams as four 4-byte words (beam quads
+        // fastest, so a wave reads whole 128-byte rows), transpose the 4 x 4
+        // bytes with v_perm_b32 and write four words of 4 channels.
+        const int nbq = bv >> 2, nfg = fv >> 2;
+        const int nitems = nbq * tv * nfg;
+        for (int i = threadIdx.x; i < nitems; i += 256) {
+            const int bq = i % nbq;
+            const int rest = i / nbq;
+            const int t = rest % tv;
+            const int fg = rest / tv;
+            const int8_t* src = a.src + ((size_t)(f0 + 4 * fg) * a.To + tg0 + t) * a.B + b0 + 4 * bq;
+            const size_t fstep = (size_t)a.To * a.B;
+            const uint32_t r0 = *reinterpret_cast<const uint32_t*>(src);
+            const uint32_t r1 = *reinterpret_cast<const uint32_t*>(src + fstep);
+            const uint32_t r2 = *reinterpret_cast<const uint32_t*>(src + 2 * fstep);
+            const uint32_t r3 = *reinterpret_cast<const uint32_t*>(src + 3 * fstep);
+            const uint32_t t0 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);   // r0.0 r1.0 r0.1 r1.1
+            const uint32_t t1 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);   // r0.2 r1.2 r0.3 r1.3
+            const uint32_t t2 = __builtin_amdgcn_perm(r3, r2, 0x05010400u);   // r2.0 r3.0 r2.1 r3.1
+            const uint32_t t3 = __builtin_amdgcn_perm(r3, r2, 0x07030602u);   // r2.2 r3.2 r2.3 r3.3
+            uint32_t* l = reinterpret_cast<uint32_t*>(tr_lds) + (size_t)t * a.tp + (size_t)(4 * bq) * a.bp + fg;
+            l[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);               // beam 0: channels 0..3
+            l[a.bp] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);            // beam 1
+            l[2 * a.bp] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);        // beam 2
+            l[3 * a.bp] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);        // beam 3
+        }
+    } else {
     // Load: (piece of 8 beams, t, f), pieces fastest.
     const int nb8 = bv >> 3;
     const int nload = fv * tv * nb8;
@@ -1275,6 +1303,7 @@ __global__ __launch_bounds__(256) void tbtf_k(TbtfArgs a)
             l[k * bstep] = (int8_t)(v.x >> (8 * k));
             l[(k + 4) * bstep] = (int8_t)(v.y >> (8 * k));
         }
+    }
     }
     __syncthreads();
 

@@ -369,6 +369,11 @@ class CbPlan:
     def set_variant(self, v):
         call("fbf_cb_plan_set_variant", self.handle, int(v))
 
+    def set_levels(self, offsets, scales):
+        off = np.ascontiguousarray(offsets, dtype=np.float32)
+        sc = np.ascontiguousarray(scales, dtype=np.float32)
+        call("fbf_cb_plan_set_levels", self.handle, _np_ptr(off), _np_ptr(sc))
+
     def output_bytes(self, ftpa_bytes):
         n = C.c_size_t()
         call("fbf_cb_output_bytes", self.handle, ftpa_bytes, C.byref(n))

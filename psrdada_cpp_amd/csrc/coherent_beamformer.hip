@@ -34,6 +34,7 @@
 // A <= 128) and handed to the quantiser as a double, like the oracle.
 #include "fbf_internal.hpp"
 
+#include <climits>
 #include <map>
 #include <mutex>
 #include <string>
@@ -59,6 +60,7 @@ struct CbArgs {
     void* out;
     const float* offsets;
     const float* scales;
+    const long long* thr;   // quantiser thresholds [Fo][kThr] (cb_thresholds_k)
     long long nsamples;   // T
     long long nsamples_out;
     int A, B, F, Fo, ts, fs, ohs, quant;
@@ -776,14 +778,6 @@ struct CbTrArgs {
     int n_bgroups, n_tgroups;
 };
 
-// Value of lane ^ 1 (DPP quad_perm [1, 0, 3, 2]).
-__device__ __forceinline__ long long xor1_64(long long v)
-{
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(unsigned long long)v, 0xB1, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)((unsigned long long)v >> 32), 0xB1, 0xF, 0xF, false);
-    return (long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo);
-}
-
 // a + b' where lanes below 32 keep a and take b' = (a of lane + 32), and
 // lanes from 32 keep b and take (b of lane - 32): the halves' exchange of a
 // reduce-scatter in one v_permlane32_swap per dword.  With a = value for the
@@ -799,40 +793,92 @@ __device__ __forceinline__ long long swap32_sum(long long a, long long b)
     return x + y;
 }
 
-// Quantiser without the IEEE division in the common case.  With
-// inv = RN(1 / scale), y = RN((P - offset) * inv) is within 2^-43 of the
-// exactly rounded quotient the oracle computes for |y| <= 256, so when y is
-// more than 2^-40 away from every rounding boundary (half-integers when
-// rounding, integers when truncating) it rounds to the same integer; values
-// beyond +-256 saturate either way.  Otherwise (and for a zero, subnormal or
-// non-finite scale) the exact quantise() runs.
+// Offset and reciprocal scale of the threshold quantiser's estimate.
 struct QuantConst {
-    float off, sc;
     double off_d, inv;
-    bool fast;
 };
 
 __device__ __forceinline__ QuantConst quant_const(float off, float sc)
 {
-    QuantConst q;
-    q.off = off;
-    q.sc = sc;
-    q.off_d = (double)off;
-    q.inv = 1.0 / (double)sc;
-    const float a = fabsf(sc);
-    q.fast = a >= 1.17549435e-38f && a <= 3.40282347e+38f && fabs(q.inv) <= 1.79769e308;
-    return q;
+    return QuantConst{(double)off, 1.0 / (double)sc};
 }
 
-__device__ __forceinline__ int8_t quantise_fast(long long P, const QuantConst& q, int mode)
+// Quantiser thresholds.  For a positive, normal, finite scale and a finite
+// offset, quantise((double)P, offset, scale, mode) is a non-decreasing step
+// function of the integer power P: thr[j] is the least P >= 0 that quantises
+// to at least j - 128 (thr[0] = INT64_MIN, thr[256] = INT64_MAX, and
+// INT64_MAX where no P < 2^62 gets there).  The kernel's estimate
+// RN((P - offset) * RN(1 / scale)) is within 2^-43 of the quotient the exact
+// quantiser rounds (|y| <= 256; beyond, both saturate alike), so it lands on
+// the right step or a neighbour, and the two thresholds around it settle the
+// result without a division or a branch.
+constexpr int kThr = 257;
+
+__global__ __launch_bounds__(256) void cb_thresholds_k(const float* offsets, const float* scales, int mode,
+                                                       long long* thr)
+{
+    const int fo = blockIdx.x;
+    const float off = offsets[fo], sc = scales[fo];
+    long long* t = thr + (size_t)fo * kThr;
+    const int j = threadIdx.x + 1;   // 1 .. 256
+    if (j == 256) {
+        t[0] = LLONG_MIN;
+        t[256] = LLONG_MAX;
+        return;
+    }
+    const int k = j - 128;
+    long long lo = 0, hi = 1ll << 62;
+    if (quantise((double)hi, off, sc, mode) < k) {
+        t[j] = LLONG_MAX;
+        return;
+    }
+    while (lo < hi) {   // least P in [lo, hi] with quantise(P) >= k
+        const long long mid = lo + (hi - lo) / 2;
+        if (quantise((double)mid, off, sc, mode) >= k)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    t[j] = lo;
+}
+
+__device__ __forceinline__ bool thr_applies(float off, float sc)
+{
+    return isfinite(off) && sc >= 1.17549435e-38f && sc <= 3.40282347e+38f;
+}
+
+__device__ __forceinline__ int8_t quantise_thr(long long P, const QuantConst& q, const long long* thr, int mode)
 {
     const double y = ((double)P - q.off_d) * q.inv;
-    const double r = rint(y);
-    const double dist = mode ? fabs(y - r) : fabs(fabs(y - floor(y)) - 0.5);
-    if (!q.fast || !(dist > 0x1p-40)) return quantise((double)P, q.off, q.sc, mode);
-    double k = mode ? trunc(y) : r;
-    k = fmin(fmax(k, -128.0), 127.0);
-    return (int8_t)(int)k;
+    double e = mode ? trunc(y) : rint(y);
+    e = fmin(fmax(e, -128.0), 127.0);
+    int j = (int)e + 128;
+    const long long lo = thr[j], hi = thr[j + 1];
+    j += (int)(P >= hi) - (int)(P < lo);
+    return (int8_t)(j - 128);
+}
+
+// One tile step of the transposed kernels: the RT tiles' MFMA chains and the
+// detection of their values into the pass accumulators pw[RT / 2][NSLOT]
+// (tile r adds to beam group r / 2; values of window q of the tile go to
+// slot g * SPT + q), one v_mad_i64_i32 per value.  (Splitting the
+// dependent chains into independent partial sums measured 0.7 % slower:
+// the other waves of the SIMD already cover their latency.)
+template <int RT, int KSTEPS, int SPT, int NSLOT>
+__device__ __forceinline__ void detect_step(const v4i (&a)[KSTEPS], const v4i (&w)[RT][KSTEPS],
+                                            long long (&pw)[RT / 2][NSLOT], int g)
+{
+    constexpr int n = 16 / SPT;   // values per window per tile
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        v16i acc = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], w[r][s], acc, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < SPT; ++q)
+#pragma unroll
+            for (int i = 0; i < n; ++i) pw[r >> 1][g * SPT + q] += (long long)acc[q * n + i] * acc[q * n + i];
+    }
 }
 
 // Staging constants of cb_tr_k for FS channels x KSTEPS 32-byte K steps.
@@ -853,7 +899,8 @@ struct TrRed {
     static constexpr int VALS = RT == 2 ? NSLOT / 2 : NSLOT;   // totals per lane after the reduce-scatter
 };
 
-// RT: beam tiles per wave; SPT: windows per tile (1, 2, 4); NSLOT: windows
+// RT: beam tiles per wave (RT / 2 groups of 32 beams, a Re and an Im tile
+// each); SPT: windows per tile (1, 2, 4); NSLOT: windows
 // per pass held in registers; FS: fscrunch; NTS: time slots per work-group.
 //
 // A work-group has 4 * NTS waves: NTS time slots (contiguous runs of passes
@@ -870,6 +917,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     constexpr int PR = 2 * KSTEPS;                 // 16-byte pieces per LDS column
     constexpr int VALS = TrRed<RT, NSLOT>::VALS;
     constexpr int NB = St::NB;
+    constexpr int NG = RT / 2;                     // groups of 32 beams per wave
     static_assert(RT == 2 || RT == 4, "RT");
     static_assert(NSLOT % SPT == 0 && (RT == 4 || NSLOT % 2 == 0), "slots");
     static_assert(St::M <= 2, "at most two staging loads per wave per set");
@@ -882,7 +930,6 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     const int wv = __builtin_amdgcn_readfirstlane(wave);
     const int h = lane >> 5;
     const int c = lane & 31;
-    const int part = c & 1;
     const int tsl = wv >> 2;         // time slot
     const int wc = (wv & 3) % FS;    // channel of the fscrunch group
     const int wb = (wv & 3) / FS;    // beam group within the work-group
@@ -902,21 +949,24 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     const int slot_tiles = ta.npasses * ta.ntp;
     const int tile0 = (tgroup * NTS + tsl) * slot_tiles;
 
-    // Weights (B operand): embedding row n = c of beam tile r.
+    // Start-up loads first, all in flight together: the channel's quantiser
+    // thresholds (to LDS below), the weights, then the first staged tiles.
+    static_assert(256 * NTS >= kThr, "one threshold per thread");
+    const long long thr_v = tid < kThr ? p.thr[(size_t)fo * kThr + tid] : 0;
+
+    // Weights (B operand).  Tile r holds the Re (r even) or Im (r odd)
+    // embedding rows of the 32 beams beam_base + 32 (r / 2) + n, n = lane %
+    // 32, so a lane's Re and Im outputs belong to one beam and its power
+    // sums in-lane.
     v4i w[RT][KSTEPS];
 #pragma unroll
-    for (int r = 0; r < RT; ++r) {
-        const int b = beam_base + r * 16 + (c >> 1);
+    for (int g = 0; g < NG; ++g) {
+        const int b = beam_base + 32 * g + c;
         const bool bvalid = b < p.B;
         const int8_t* wcol = p.fba + ((size_t)f * p.B + (bvalid ? b : p.B - 1)) * K;
 #pragma unroll
-        for (int s = 0; s < KSTEPS; ++s) w[r][s] = load_wslice<true>(wcol, 16 * (h * KSTEPS + s), K, bvalid);
+        for (int s = 0; s < KSTEPS; ++s) w[2 * g][s] = load_wslice<true>(wcol, 16 * (h * KSTEPS + s), K, bvalid);
     }
-    // Every load is in flight before the first is used.
-#pragma unroll
-    for (int r = 0; r < RT; ++r)
-#pragma unroll
-        for (int s = 0; s < KSTEPS; ++s) w[r][s] = embed_slice(w[r][s], part);
 
     // Staging.  A wave's 16-byte load lands at LDS offset 16 * lane of a
     // 1 KiB run, so the bank swizzle is chosen on the global side: physical
@@ -926,6 +976,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     const int kpieces = K / 16;
     int8_t* dummy = lds + NB * St::SET;
     long long* red = reinterpret_cast<long long*>(dummy + St::DUMMY);   // [waves][VALS][64]
+    long long* thr = red + (FS > 1 ? St::NW * VALS * 64 : 0);           // [kThr] of channel fo
     // A set holds one tile per (time slot, channel): tile index
     // tsl * FS + ch.  This lane's loads: source offset less the tile's
     // column offset, the first tile of the slot they serve, LDS destination.
@@ -978,13 +1029,41 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     int rs = 0;              // ring slot read this step
     int is = NB - St::G;     // next ring slot to fill
 
-    const QuantConst qz = quant_const(p.offsets[fo], p.scales[fo]);
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) {
+            w[2 * g + 1][s] = embed_slice(w[2 * g][s], true);
+            w[2 * g][s] = embed_slice(w[2 * g][s], false);
+        }
+
+    // Quantiser: the thresholds are visible after the tile loop's first
+    // barrier.
+    const float q_off = p.offsets[fo], q_sc = p.scales[fo];
+    const bool q_thr = thr_applies(q_off, q_sc);
+    const QuantConst qz = quant_const(q_off, q_sc);
+    if (tid < kThr) thr[tid] = thr_v;
+    // Quantise and store one pass's totals (staged [fo][to][b] int8); with
+    // FS > 1 the waves of a beam group share the totals out.
+    const int b_out = RT == 2 ? beam_base + c : beam_base + 32 * h + c;
+    int8_t* const out_f = static_cast<int8_t*>(p.out) + (size_t)fo * p.nsamples_out * p.B + b_out;
+    auto store_pass = [&](const long long (&t)[VALS], int w0) {
+#pragma unroll
+        for (int v = 0; v < VALS; ++v) {
+            if (FS > 1 && v % FS != wc) continue;
+            const int sl = RT == 2 ? 2 * v + h : v;
+            const long long to = (long long)w0 + sl;
+            if (!wave_active || b_out >= p.B || to >= p.nsamples_out) continue;
+            out_f[(size_t)to * p.B] = q_thr ? quantise_thr(t[v], qz, thr, p.quant)
+                                            : quantise((double)t[v], q_off, q_sc, p.quant);
+        }
+    };
     for (int pass = 0; pass < ta.npasses; ++pass) {
-        long long pw[RT][NSLOT];
+        long long pw[NG][NSLOT];
 #pragma unroll
-        for (int r = 0; r < RT; ++r)
+        for (int g = 0; g < NG; ++g)
 #pragma unroll
-            for (int sl = 0; sl < NSLOT; ++sl) pw[r][sl] = 0;
+            for (int sl = 0; sl < NSLOT; ++sl) pw[g][sl] = 0;
 
         // Tiles of the pass: NSLOT/SPT groups of tps tiles (tps = 1 unless
         // a window spans several tiles).
@@ -1009,52 +1088,23 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
                     v4i a[KSTEPS];
 #pragma unroll
                     for (int s = 0; s < KSTEPS; ++s) a[s] = *reinterpret_cast<const v4i*>(src + aoff[s]);
-#pragma unroll
-                    for (int r = 0; r < RT; ++r) {
-                        v16i acc = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-                        for (int s = 0; s < KSTEPS; ++s)
-                            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], w[r][s], acc, 0, 0, 0);
-                        // One v_mad_i64_i32 per value straight into the
-                        // window's accumulator (the RT chains and the other
-                        // waves of the SIMD cover the dependency latency).
-#pragma unroll
-                        for (int q = 0; q < SPT; ++q) {
-                            constexpr int n = 16 / SPT;
-#pragma unroll
-                            for (int i = 0; i < n; ++i)
-                                pw[r][g * SPT + q] += (long long)acc[q * n + i] * acc[q * n + i];
-                        }
-                    }
+                    detect_step<RT, KSTEPS, SPT, NSLOT>(a, w, pw, g);
                 }
                 rs = rs + 1 == NB ? 0 : rs + 1;
                 ++step;
             }
         }
 
-        // Reduce-scatter over {Re, Im} (lane ^ 1, DPP) and row halves
-        // (lane ^ 32, v_permlane32_swap): no LDS round trips.
+        // A window's power is split over the row halves (lane and lane ^ 32):
+        // one v_permlane32_swap reduce-scatter per value, lanes below 32
+        // keeping beam group 0 (RT = 4) or the even slots (RT = 2).
         long long tot[VALS];
         if constexpr (RT == 2) {
-            // lane keeps r = part: sum with lane ^ 1, then slots 2j + h
-            long long t1[NSLOT];
 #pragma unroll
-            for (int sl = 0; sl < NSLOT; ++sl) {
-                const long long u0 = pw[0][sl] + xor1_64(pw[0][sl]);
-                const long long u1 = pw[1][sl] + xor1_64(pw[1][sl]);
-                t1[sl] = part ? u1 : u0;
-            }
-#pragma unroll
-            for (int j = 0; j < VALS; ++j) tot[j] = swap32_sum(t1[2 * j], t1[2 * j + 1]);
+            for (int v = 0; v < VALS; ++v) tot[v] = swap32_sum(pw[0][2 * v], pw[0][2 * v + 1]);
         } else {
-            // lane keeps r = part + 2h
 #pragma unroll
-            for (int sl = 0; sl < NSLOT; ++sl) {
-                long long u[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) u[r] = pw[r][sl] + xor1_64(pw[r][sl]);
-                tot[sl] = swap32_sum(part ? u[1] : u[0], part ? u[3] : u[2]);
-            }
+            for (int sl = 0; sl < NSLOT; ++sl) tot[sl] = swap32_sum(pw[0][sl], pw[1][sl]);
         }
 
         // Sum over the fscrunch channels (one per wave of a beam group).
@@ -1074,20 +1124,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
             block_barrier();
         }
 
-        // Quantise and store (staged [fo][to][b] int8); with FS > 1 the waves
-        // of a beam group share the totals out.
-        const int win0 = (tile0 + pass * ta.ntp) * SPT / ta.tps;
-        const int r = RT == 2 ? part : part + 2 * h;
-        const int b = beam_base + r * 16 + (c >> 1);
-#pragma unroll
-        for (int v = 0; v < VALS; ++v) {
-            if (FS > 1 && v % FS != wc) continue;
-            const int sl = RT == 2 ? 2 * v + h : v;
-            const long long to = (long long)win0 + sl;
-            if (!wave_active || b >= p.B || to >= p.nsamples_out) continue;
-            static_cast<int8_t*>(p.out)[((size_t)fo * p.nsamples_out + (size_t)to) * p.B + b] =
-                quantise_fast(tot[v], qz, p.quant);
-        }
+        store_pass(tot, (tile0 + pass * ta.ntp) * SPT / ta.tps);
     }
     // The loads past the last tile are still in flight: LDS must not be
     // released before they land.
@@ -1100,8 +1137,8 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 // output.
 inline int cb_tr_nslot(int rt, int W)
 {
-    if (W == 8) return 4;
-    return rt == 2 ? 4 : 2;
+    if (W <= 16) return rt == 2 ? 8 : 4;
+    return W == 32 ? 4 : 2;
 }
 
 inline int cb_tr_nb(int ni) { return 60 / ni < 2 ? 2 : (60 / ni > 8 ? 8 : 60 / ni); }
@@ -1114,7 +1151,7 @@ inline size_t cb_tr_shm(int fs, int ksteps, int vals)
     const int nw = 4 * kTrNts;
     const int ni = kTrNts * fs * ksteps;
     return (size_t)cb_tr_nb(ni) * ni * 1024 + ((ni % nw) ? (size_t)nw * 1024 : 0) +
-           (fs > 1 ? (size_t)nw * vals * 64 * sizeof(long long) : 0);
+           (fs > 1 ? (size_t)nw * vals * 64 * sizeof(long long) : 0) + kThr * sizeof(long long);
 }
 
 inline bool cb_tr_supported(int A, int ts, int fs, int rt)
@@ -1205,10 +1242,12 @@ hipError_t launch_tr_ks(const CbTrArgs& a, int ksteps, int passes, hipStream_t s
 template <int RT, int FS>
 hipError_t launch_tr_w(const CbTrArgs& a, int W, int ksteps, int passes, hipStream_t s)
 {
-    constexpr int base = RT == 2 ? 4 : 2;
-    if (W == 8) return launch_tr_ks<RT, 4, 4, FS>(a, ksteps, passes, s);
-    if (W == 16) return launch_tr_ks<RT, 2, base, FS>(a, ksteps, passes, s);
-    return launch_tr_ks<RT, 1, base, FS>(a, ksteps, passes, s);
+    // Windows per pass as cb_tr_nslot
+    constexpr int small = RT == 2 ? 8 : 4;
+    if (W == 8) return launch_tr_ks<RT, 4, small, FS>(a, ksteps, passes, s);
+    if (W == 16) return launch_tr_ks<RT, 2, small, FS>(a, ksteps, passes, s);
+    if (W == 32) return launch_tr_ks<RT, 1, 4, FS>(a, ksteps, passes, s);
+    return launch_tr_ks<RT, 1, 2, FS>(a, ksteps, passes, s);
 }
 
 template <int RT>
@@ -1217,6 +1256,337 @@ hipError_t launch_tr(const CbTrArgs& a, int fs, int W, int ksteps, int passes, h
     if (fs == 1) return launch_tr_w<RT, 1>(a, W, ksteps, passes, s);
     if (fs == 2) return launch_tr_w<RT, 2>(a, W, ksteps, passes, s);
     return launch_tr_w<RT, 4>(a, W, ksteps, passes, s);
+}
+
+// ---------------------------------------------------------------------------
+// Wave-private staging (cb_ws_k)
+//
+// The contraction, detection, reduce-scatter and quantiser of cb_tr_k with
+// RT = 4 (a wave holds 64 beams, Re and Im tiles) and fscrunch 1, but every
+// wave stages its own voltage tiles in a private LDS ring and waits for
+// nothing but its own loads: no work-group barrier in the tile loop, so the
+// waves of a SIMD drift apart and one wave's MFMAs overlap another wave's
+// detection (the 64-bit multiply-adds take as long as the MFMAs at 64
+// antennas).  Work is dealt dynamically.  A unit is (output channel, 64-beam
+// group, time range); a work-group of 12 waves serves U units, the waves
+// starting 12 / U to a unit; a wave takes chunks (whole passes, at least a
+// ring's worth of tiles) from its unit's LDS counter and, when the unit runs
+// dry, moves to the next unit of the work-group (reloading its weights), so
+// the waves of a work-group end within a chunk of each other.
+//
+// Ring: NB tiles per wave; the tile at sequence position q sits in slot
+// q % NB.  Step q waits for its own loads of tile q (a fixed s_waitcnt:
+// (NB - 1) tiles issued after it), reads the tile into registers, and then
+// reuses the slot for tile q + NB.  A slot is written only after the same
+// wave's reads of it have completed, and the loads are issued for every
+// step (past the end of the work they re-read a valid tile), so the count
+// stays exact.
+// ---------------------------------------------------------------------------
+// Returns the old value of the LDS word at byte address addr and adds v.
+// Written as one asm block that also waits for the result: the compiler
+// puts a full vmcnt(0) wait before its own LDS atomics while global->LDS
+// loads are in flight, which would drain a wave's staging ring at every
+// chunk.
+__device__ __forceinline__ int lds_fetch_add(unsigned addr, int v)
+{
+    int old;
+    __asm__ volatile("ds_add_rtn_u32 %0, %1, %2\n\ts_waitcnt lgkmcnt(0)" : "=v"(old) : "v"(addr), "v"(v) : "memory");
+    return old;
+}
+
+constexpr int kWsWaves = 12;
+constexpr int kWsRingBudget = 144 * 1024;
+constexpr int kWsMaxUnits = 6;
+
+template <int KSTEPS>
+struct WsStage {
+    static constexpr int KPAD = 32 * KSTEPS;
+    static constexpr int TILE = 32 * KPAD;                  // bytes per tile
+    static constexpr int LPT = KSTEPS;                      // 1 KiB loads per tile
+    static constexpr int NB0 = kWsRingBudget / (kWsWaves * TILE);
+    static constexpr int NB = NB0 < 2 ? 2 : (NB0 > 8 ? 8 : NB0);
+    static constexpr int WAIT = (NB - 1) * LPT;
+};
+
+inline int cb_ws_nb(int ksteps)
+{
+    const int nb = kWsRingBudget / (kWsWaves * 1024 * ksteps);
+    return nb < 2 ? 2 : (nb > 8 ? 8 : nb);
+}
+
+struct CbWsArgs {
+    CbArgs p;
+    int ntiles;      // 32-column tiles per channel
+    int tps;         // tiles per window (W >= 32), else 1
+    int ntp;         // tiles per pass
+    int ct;          // tiles per chunk: a whole number of passes, >= NB
+    int nbg;         // 64-beam groups
+    int ntr;         // time ranges per (channel, beam group)
+    int tr_chunks;   // chunks per time range
+    int nunits;      // Fo * nbg * ntr
+    int U;           // units per work-group
+};
+
+template <int KSTEPS, int SPT, int NSLOT>
+__global__ __launch_bounds__(64 * kWsWaves) void cb_ws_k(CbWsArgs wa)
+{
+    using St = WsStage<KSTEPS>;
+    constexpr int KPAD = St::KPAD;
+    constexpr int PR = 2 * KSTEPS;
+    constexpr int NB = St::NB;
+    constexpr int RT = 4, NG = 2;
+    static_assert(NSLOT % SPT == 0, "slots");
+    static_assert(St::WAIT <= 63, "vmcnt");
+    extern __shared__ __align__(16) int8_t lds[];
+    const CbArgs& p = wa.p;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int h = lane >> 5;
+    const int c = lane & 31;
+    const int U = wa.U;
+    const int unit0 = xcd_remap(blockIdx.x, gridDim.x) * U;
+    int8_t* const ring = lds + wv * NB * St::TILE;
+    long long* const thr_all = reinterpret_cast<long long*>(lds + kWsWaves * NB * St::TILE);   // [U][kThr]
+    int* const counter = reinterpret_cast<int*>(thr_all + U * kThr);                          // [U]
+
+    // Units: id = ((fo * ntr + tr) * nbg + bg).
+    auto unit_fo = [&](int gu) { return gu / (wa.ntr * wa.nbg); };
+    for (int i = tid; i < U * kThr; i += 64 * kWsWaves) {
+        const int gu = unit0 + i / kThr;
+        thr_all[i] = gu < wa.nunits ? p.thr[(size_t)unit_fo(gu) * kThr + i % kThr] : 0;
+    }
+    if (tid < U) counter[tid] = 0;
+    __syncthreads();
+
+    const int K = 2 * p.A;
+    const size_t TP = (size_t)2 * p.nsamples;
+    const int kpieces = K / 16;
+    const int chunk_tiles = wa.ct;
+
+    // This lane's part of a tile load m: column and K piece (swizzled as in
+    // cb_tr_k: physical piece pp of column col holds K piece
+    // (pp + col * KPAD / 256) % PR; pieces past K re-read piece 0, whose
+    // weights there are zero).
+    int goff[St::LPT];
+#pragma unroll
+    for (int m = 0; m < St::LPT; ++m) {
+        const int off = m * 1024 + lane * 16;
+        const int col = off / KPAD;
+        const int pp = (off - col * KPAD) >> 4;
+        const int j = (pp + ((col * KPAD) >> 8)) % PR;
+        goff[m] = col * p.vstride + 16 * (j < kpieces ? j : 0);
+    }
+    int aoff[KSTEPS];
+    {
+        const int rot = (c * KPAD) >> 8;
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) aoff[s] = c * KPAD + 16 * ((h * KSTEPS + s - rot + 32 * PR) % PR);
+    }
+
+    // Chunk descriptors: (local unit << 24) | chunk, or -1 once every unit
+    // of the work-group is dry.
+    int home = (wv * U) / kWsWaves;
+    const unsigned counter_lds = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)counter;
+    auto grab = [&]() -> int {
+        for (int k = 0; k < U; ++k) {
+            int got = 0;
+            if (lane == 0) got = lds_fetch_add(counter_lds + 4u * (unsigned)home, 1);
+            got = __builtin_amdgcn_readfirstlane(got);
+            const int gu = unit0 + home;
+            const int tr = (gu / wa.nbg) % wa.ntr;
+            if (gu < wa.nunits && got < wa.tr_chunks && (tr * wa.tr_chunks + got) * chunk_tiles < wa.ntiles)
+                return (home << 24) | got;
+            home = home + 1 == U ? 0 : home + 1;
+        }
+        return -1;
+    };
+    // First tile (in its channel) and channel byte offset of a chunk.
+    auto chunk_tile0 = [&](int d) {
+        const int gu = unit0 + (d >> 24);
+        const int tr = (gu / wa.nbg) % wa.ntr;
+        return (tr * wa.tr_chunks + (d & 0xFFFFFF)) * chunk_tiles;
+    };
+    // The issue cursor's chunk as a first-tile address and a count of tiles
+    // inside the channel (later tiles re-read the last one; past the end of
+    // the work every load re-reads tile 0 of channel 0).
+    const size_t tile_bytes = (size_t)32 * p.vstride;
+    const int8_t* isrc = p.ftpa + p.voff;
+    int ilast = 0;
+    auto set_issue_chunk = [&](int d) {
+        isrc = p.ftpa + p.voff;
+        ilast = 0;
+        if (d >= 0) {
+            const int t0 = chunk_tile0(d);
+            isrc += (size_t)unit_fo(unit0 + (d >> 24)) * TP * p.vstride + (size_t)t0 * tile_bytes;
+            ilast = (wa.ntiles - t0 < chunk_tiles ? wa.ntiles - t0 : chunk_tiles) - 1;
+        }
+    };
+    auto issue = [&](int off, int slot) {
+        const int8_t* src = isrc + (size_t)(off < ilast ? off : ilast) * tile_bytes;
+#pragma unroll
+        for (int m = 0; m < St::LPT; ++m) dma16(src + (uint32_t)goff[m], ring + slot * St::TILE + m * 1024);
+    };
+
+    int cdesc = grab();          // chunk of the compute cursor
+    int ioff = 0;                // issue cursor's tile within its chunk
+    int pend = -1;               // next chunk, taken by the issue cursor
+    set_issue_chunk(cdesc);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        issue(ioff, q);
+        ++ioff;
+    }
+
+    // Weights of the unit being computed (loaded when it changes).
+    v4i w[RT][KSTEPS];
+    int wunit = -1;
+    int fo = 0, beam_base = 0;
+    float q_off = 0.f, q_sc = 1.f;
+    bool q_thr = false;
+    QuantConst qz{0.0, 1.0};
+    const long long* thr = thr_all;
+
+    int q = 0;   // sequence position of the compute cursor
+    while (cdesc >= 0) {
+        const int u = cdesc >> 24;
+        if (u != wunit) {
+            wunit = u;
+            const int gu = unit0 + u;
+            fo = unit_fo(gu);
+            beam_base = (gu % wa.nbg) * 64;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const int b = beam_base + 32 * g + c;
+                const bool bvalid = b < p.B;
+                const int8_t* wcol = p.fba + ((size_t)fo * p.B + (bvalid ? b : p.B - 1)) * K;
+#pragma unroll
+                for (int s = 0; s < KSTEPS; ++s)
+                    w[2 * g][s] = load_wslice<true>(wcol, 16 * (h * KSTEPS + s), K, bvalid);
+            }
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+#pragma unroll
+                for (int s = 0; s < KSTEPS; ++s) {
+                    w[2 * g + 1][s] = embed_slice(w[2 * g][s], true);
+                    w[2 * g][s] = embed_slice(w[2 * g][s], false);
+                }
+            q_off = p.offsets[fo];
+            q_sc = p.scales[fo];
+            q_thr = thr_applies(q_off, q_sc);
+            qz = quant_const(q_off, q_sc);
+            thr = thr_all + u * kThr;
+        }
+        const int ctile0 = chunk_tile0(cdesc);
+        const int b_out = beam_base + 32 * h + c;
+        const bool wave_active = beam_base < p.B;
+
+        for (int pc = 0; pc < chunk_tiles; pc += wa.ntp) {
+            long long pw[NG][NSLOT];
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+#pragma unroll
+                for (int sl = 0; sl < NSLOT; ++sl) pw[g][sl] = 0;
+#pragma unroll
+            for (int g = 0; g < NSLOT / SPT; ++g) {
+                for (int x = 0; x < wa.tps; ++x) {
+                    const int slot = q % NB;
+                    wait_vm_lgkm0<St::WAIT>();
+                    const int8_t* src = ring + slot * St::TILE;
+                    v4i a[KSTEPS];
+#pragma unroll
+                    for (int s = 0; s < KSTEPS; ++s) a[s] = *reinterpret_cast<const v4i*>(src + aoff[s]);
+                    wait_vm_lgkm0<63>();
+                    // The slot is free again: refill it NB positions ahead.
+                    if (ioff == chunk_tiles) {
+                        pend = grab();
+                        set_issue_chunk(pend);
+                        ioff = 0;
+                    }
+                    issue(ioff, slot);
+                    ++ioff;
+                    detect_step<RT, KSTEPS, SPT, NSLOT>(a, w, pw, g);
+                    ++q;
+                }
+            }
+            // Reduce-scatter over the row halves (lanes below 32 keep beam
+            // group 0), quantise, store.
+            long long tot[NSLOT];
+#pragma unroll
+            for (int sl = 0; sl < NSLOT; ++sl) tot[sl] = swap32_sum(pw[0][sl], pw[1][sl]);
+            const int win0 = (ctile0 + pc) * SPT / wa.tps;
+#pragma unroll
+            for (int v = 0; v < NSLOT; ++v) {
+                const long long to = (long long)win0 + v;
+                if (!wave_active || b_out >= p.B || to >= p.nsamples_out) continue;
+                int8_t* row = static_cast<int8_t*>(p.out) + ((size_t)fo * p.nsamples_out + (size_t)to) * p.B;
+                row[b_out] = q_thr ? quantise_thr(tot[v], qz, thr, p.quant)
+                                   : quantise((double)tot[v], q_off, q_sc, p.quant);
+            }
+        }
+        cdesc = pend;
+        pend = -1;
+    }
+    // Loads issued past the last chunk are still in flight.
+    wait_vm_lgkm0<0>();
+}
+
+// Units, chunks and units per work-group of cb_ws_k for one launch.
+template <int KSTEPS, int SPT, int NSLOT>
+hipError_t launch_ws_k(CbWsArgs a, hipStream_t s)
+{
+    const int nb = WsStage<KSTEPS>::NB;
+    a.ct = a.ntp * ((nb + a.ntp - 1) / a.ntp);
+    const int chunks = (a.ntiles + a.ct - 1) / a.ct;
+    const long long base_units = (long long)a.p.Fo * a.nbg;
+    const int ncu = device_cu_count();
+    int ntr = 1;
+    if (base_units < ncu) ntr = (int)((ncu + base_units - 1) / base_units);
+    if (ntr > chunks) ntr = chunks;
+    a.tr_chunks = (chunks + ntr - 1) / ntr;
+    a.ntr = (chunks + a.tr_chunks - 1) / a.tr_chunks;
+    const long long nunits = base_units * a.ntr;
+    if (nunits >= (1ll << 24)) return hipErrorInvalidConfiguration;
+    a.nunits = (int)nunits;
+    static const int kU[] = {1, 2, 3, 4, 6};
+    a.U = kWsMaxUnits;
+    for (int u : kU)
+        if ((nunits + u - 1) / u <= ncu) {
+            a.U = u;
+            break;
+        }
+    const long long nblocks = (nunits + a.U - 1) / a.U;
+    const size_t shm = (size_t)kWsWaves * nb * WsStage<KSTEPS>::TILE + (size_t)a.U * (kThr * sizeof(long long) + 4);
+    cb_ws_k<KSTEPS, SPT, NSLOT><<<dim3((unsigned)nblocks), 64 * kWsWaves, shm, s>>>(a);
+    return hipGetLastError();
+}
+
+template <int SPT, int NSLOT>
+hipError_t launch_ws_ks(const CbWsArgs& a, int ksteps, hipStream_t s)
+{
+    switch (ksteps) {
+    case 1: return launch_ws_k<1, SPT, NSLOT>(a, s);
+    case 2: return launch_ws_k<2, SPT, NSLOT>(a, s);
+    case 3: return launch_ws_k<3, SPT, NSLOT>(a, s);
+    case 4: return launch_ws_k<4, SPT, NSLOT>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Windows per pass as cb_tr_nslot with RT = 4.
+inline hipError_t launch_ws(const CbWsArgs& a, int W, int ksteps, hipStream_t s)
+{
+    if (W == 8) return launch_ws_ks<4, 4>(a, ksteps, s);
+    if (W == 16) return launch_ws_ks<2, 4>(a, ksteps, s);
+    if (W == 32) return launch_ws_ks<1, 4>(a, ksteps, s);
+    return launch_ws_ks<1, 2>(a, ksteps, s);
+}
+
+// cb_ws_k serves 8 | A <= 64, fscrunch 1, tscrunch a power of two >= 4.
+inline bool cb_ws_supported(int A, int ts, int fs)
+{
+    return A % 8 == 0 && A <= 64 && fs == 1 && ts >= 4 && (ts & (ts - 1)) == 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1381,10 +1751,15 @@ struct fbf_cb_plan {
     int ohs;
     int variant;
     int chunk;        // LDS chunk length in columns, 0 = automatic
-    float* d_levels;  // device: offsets[Fo] then scales[Fo]
+    float* d_levels;  // device: offsets[Fo], scales[Fo], then the thresholds [Fo][kThr] (int64)
     void* staging;    // [Fo][To][B] int8 between the beamformer and tbtf_k
     size_t staging_bytes;
 };
+
+static long long* cb_thr(const fbf_cb_plan* plan)
+{
+    return reinterpret_cast<long long*>(plan->d_levels + 2 * (plan->d.nchans / plan->d.fscrunch));
+}
 
 // Grows the plan's staging buffer (not while the stream is being captured).
 static int cb_staging(fbf_cb_plan* plan, size_t bytes, hipStream_t s, void** out)
@@ -1447,7 +1822,7 @@ extern "C" int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets,
     int ohs = desc->nsamples_per_heap_out ? desc->nsamples_per_heap_out : 8192 / Fo;
     FBF_REQUIRE(ohs > 0, FBF_ERR_ARG, "fbf_cb_plan_create: output heap holds no samples");
     auto* p = new fbf_cb_plan{*desc, ohs, 0, 0, nullptr, nullptr, 0};
-    hipError_t e = hipMalloc(&p->d_levels, sizeof(float) * 2 * Fo);
+    hipError_t e = hipMalloc(&p->d_levels, sizeof(float) * 2 * Fo + sizeof(long long) * kThr * Fo);
     if (e != hipSuccess) {
         delete p;
         return hip_status(e, "hipMalloc(levels)");
@@ -1476,6 +1851,10 @@ extern "C" int fbf_cb_plan_set_levels(fbf_cb_plan* plan, const float* offsets, c
     const int Fo = plan->d.nchans / plan->d.fscrunch;
     FBF_HIP_TRY(hipMemcpy(plan->d_levels, offsets, sizeof(float) * Fo, hipMemcpyHostToDevice));
     FBF_HIP_TRY(hipMemcpy(plan->d_levels + Fo, scales, sizeof(float) * Fo, hipMemcpyHostToDevice));
+    cb_thresholds_k<<<Fo, 256, 0, nullptr>>>(plan->d_levels, plan->d_levels + Fo, plan->d.quant_mode,
+                                             cb_thr(plan));
+    FBF_HIP_TRY(hipGetLastError());
+    FBF_HIP_TRY(hipStreamSynchronize(nullptr));
     return FBF_OK;
 }
 
@@ -1499,27 +1878,36 @@ struct CbVariant {
     int rows;        // row tiles per wave (RT for 32x32, ROWT for 16x16)
     bool lds;        // voltage tile staged through LDS and shared by the work-group
     bool tr;         // transposed tiles (cb_tr_k): voltages as the A operand
+    bool ws;         // transposed tiles with wave-private staging (cb_ws_k)
 };
 static const CbVariant kCbVariants[] = {
-    {0, 0, false, false},   // auto
-    {32, 4, false, false},  // 64 beams x 32 samples per wave
-    {32, 2, false, false},  // 32 beams x 32 samples per wave
-    {16, 8, false, false},  // 64 beams x 16 samples per wave
-    {16, 4, false, false},  // 32 beams x 16 samples per wave
-    {32, 1, true, false},   // LDS: 16 beams per wave, 64 per work-group
-    {32, 2, true, false},   // LDS: 32 beams per wave, 128 per work-group
-    {32, 4, true, false},   // LDS: 64 beams per wave, 256 per work-group
-    {32, 2, true, true},    // transposed: 32 beams per wave
-    {32, 4, true, true},    // transposed: 64 beams per wave
+    {0, 0, false, false, false},   // auto
+    {32, 4, false, false, false},  // 64 beams x 32 samples per wave
+    {32, 2, false, false, false},  // 32 beams x 32 samples per wave
+    {16, 8, false, false, false},  // 64 beams x 16 samples per wave
+    {16, 4, false, false, false},  // 32 beams x 16 samples per wave
+    {32, 1, true, false, false},   // LDS: 16 beams per wave, 64 per work-group
+    {32, 2, true, false, false},   // LDS: 32 beams per wave, 128 per work-group
+    {32, 4, true, false, false},   // LDS: 64 beams per wave, 256 per work-group
+    {32, 2, true, true, false},    // transposed: 32 beams per wave
+    {32, 4, true, true, false},    // transposed: 64 beams per wave
+    {32, 4, true, true, true},     // transposed, wave-private staging: 64 beams per wave
 };
 constexpr int kCbVariantLds = 5;
 constexpr int kCbVariantTr2 = 8;
 constexpr int kCbVariantTr4 = 9;
+constexpr int kCbVariantWs = 10;
 
 static bool cb_tr_applies(const fbf_cb_desc& d, int rt, int mode, long long T)
 {
     return mode == MODE_INT8 && T % 16 == 0 && 2 * T / 32 < (1ll << 31) &&
            cb_tr_supported(d.nantennas, d.tscrunch, d.fscrunch, rt);
+}
+
+static bool cb_ws_applies(const fbf_cb_desc& d, int mode, long long T)
+{
+    return mode == MODE_INT8 && T % 16 == 0 && 2 * T / 32 < (1ll << 31) &&
+           cb_ws_supported(d.nantennas, d.tscrunch, d.fscrunch);
 }
 
 // Automatic choice: the transposed kernel with 64 beams per wave, else with
@@ -1533,7 +1921,8 @@ static CbVariant cb_pick_variant(const fbf_cb_desc& d, int variant, int mode, lo
         return kCbVariants[kCbVariantLds];
     }
     const CbVariant v = kCbVariants[variant];
-    if (v.tr && !cb_tr_applies(d, v.rows, mode, T)) return kCbVariants[kCbVariantLds];
+    if (v.ws && !cb_ws_applies(d, mode, T)) return cb_pick_variant(d, 0, mode, T);
+    if (v.tr && !v.ws && !cb_tr_applies(d, v.rows, mode, T)) return kCbVariants[kCbVariantLds];
     return v;
 }
 
@@ -1625,6 +2014,42 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
     }
 
     const CbVariant var = cb_pick_variant(d, plan->variant, mode, T);
+    if (var.ws) {
+        CbWsArgs wa{};
+        CbArgs& a = wa.p;
+        a.ftpa = static_cast<const int8_t*>(ftpa);
+        a.fba = static_cast<const int8_t*>(fba);
+        a.out = kout;
+        a.offsets = plan->d_levels;
+        a.scales = plan->d_levels + Fo;
+        a.thr = cb_thr(plan);
+        a.nsamples = T;
+        a.nsamples_out = To;
+        a.A = d.nantennas;
+        a.B = d.nbeams;
+        a.F = d.nchans;
+        a.Fo = Fo;
+        a.ts = d.tscrunch;
+        a.fs = d.fscrunch;
+        a.ohs = plan->ohs;
+        a.quant = d.quant_mode;
+        a.vstride = vstride;
+        a.voff = voff;
+        const int W = 2 * d.tscrunch;
+        const int spt = W >= 32 ? 1 : 32 / W;
+        wa.tps = W >= 32 ? W / 32 : 1;
+        wa.ntp = (cb_tr_nslot(4, W) / spt) * wa.tps;
+        wa.ntiles = (int)(2 * T / 32);
+        wa.nbg = (d.nbeams + 63) / 64;
+        const int ksteps = (2 * d.nantennas + 31) / 32;
+        hipError_t e = launch_ws(wa, W, ksteps, s);
+        if (e != hipSuccess) return hip_status(e, "cb_ws_k launch");
+        e = launch_tbtf(tbtf_args(static_cast<const int8_t*>(kout), static_cast<int8_t*>(out), To, d.nbeams, Fo,
+                                  plan->ohs),
+                        s);
+        if (e != hipSuccess) return hip_status(e, "tbtf_k launch");
+        return FBF_OK;
+    }
     if (var.tr) {
         CbTrArgs ta{};
         CbArgs& a = ta.p;
@@ -1633,6 +2058,7 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
         a.out = kout;
         a.offsets = plan->d_levels;
         a.scales = plan->d_levels + Fo;
+        a.thr = cb_thr(plan);
         a.nsamples = T;
         a.nsamples_out = To;
         a.A = d.nantennas;
@@ -1684,6 +2110,7 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
     a.out = kout;
     a.offsets = plan->d_levels;
     a.scales = plan->d_levels + Fo;
+    a.thr = cb_thr(plan);
     a.nsamples = T;
     a.nsamples_out = To;
     a.A = d.nantennas;

@@ -325,7 +325,7 @@ def test_weights_fp64_phase_config4(gpu, oracle, epoch):
     assert (diff > 0).mean() < 1e-3
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("F,T,A,B,ts,fs", [(4, 512, 64, 512, 16, 1), (4, 256, 44, 96, 4, 2), (2, 512, 8, 64, 16, 1),
                                             (2, 256, 40, 48, 64, 1), (2, 256, 128, 32, 8, 1),
                                             (4, 512, 40, 96, 4, 4), (4, 1024, 64, 256, 64, 2), (2, 256, 16, 48, 8, 1),
@@ -353,7 +353,7 @@ def test_cb_every_variant_bit_exact(gpu, oracle, variant, F, T, A, B, ts, fs):
     np.testing.assert_array_equal(d_p.download(np.float64).reshape(F // fs, T // ts, B), p)
 
 
-@pytest.mark.parametrize("variant", [5, 8, 9])
+@pytest.mark.parametrize("variant", [5, 8, 9, 10])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_cb_quantiser_rounding_boundaries(gpu, oracle, variant, mode):
     """Levels that put many outputs exactly on (or next to) rounding
@@ -380,3 +380,71 @@ def test_cb_quantiser_rounding_boundaries(gpu, oracle, variant, mode):
     np.testing.assert_array_equal(d_o.download(), ref)
     # the levels really produce ties: half-integer quotients in channel 0
     assert np.any(p[0] % 2 == 1)
+
+
+@pytest.mark.parametrize("F,T,A,B,ts,mode", [
+    (4, 2048, 64, 512, 16, 0),     # 8 units: time ranges, one unit per work-group
+    (37, 256, 16, 576, 16, 1),     # 333 units: two per work-group, the last work-group holds one
+    (64, 128, 8, 1536, 8, 0),      # 1536 units: six per work-group
+    (2, 4096, 56, 64, 4, 1),       # K = 112 (pieces past K), window 8: one tile per pass
+    (3, 1200, 40, 72, 8, 0),       # KSTEPS 3, a partial last chunk, a partial beam group
+    (5, 512, 24, 200, 64, 1),      # window 128: four tiles per window, two windows per pass
+    (2, 8192, 8, 64, 32, 0),       # KSTEPS 1: eight-tile rings
+])
+def test_cb_wave_private_staging_bit_exact(gpu, oracle, F, T, A, B, ts, mode):
+    """Variant 10 (wave-private staging, dynamic chunks): unit counts that put
+    one, two (with a ragged last work-group) and six units on a work-group,
+    time ranges when units are few, every K step count, window sizes from one
+    tile per pass to several tiles per window, and ragged tiles, chunks and
+    beam groups; int8 TBTF bit-exact with the oracle, also read in place
+    from a wider FTPA block (subarray view)."""
+    capi = gpu
+    rng = np.random.default_rng(5000 + F * 7 + A)
+    ftpa = gaussian_int8(5000 + F * 7 + A, F * T * 2 * A * 2, 24.0)
+    w = rng.integers(-127, 128, F * B * A * 2).astype(np.int8)
+    off, sc = _levels(oracle, A, ts, 1, F)
+    ohs = T // ts
+    plan = capi.CbPlan(A, B, F, ts, 1, off, sc, nsamples_per_heap_out=ohs, quant_mode=mode)
+    plan.set_variant(10)
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    d_o = capi.DeviceBuffer(F * (T // ts) * B)
+    plan.beamform(d_v, d_w, d_o)
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, 1)
+    ref = oracle.cb_tbtf(p, off, sc, ohs, mode)
+    np.testing.assert_array_equal(d_o.download(), ref)
+    # the same antennas as a slice of a wider block
+    A2, a0 = A + 16, 8
+    blk = rng.integers(-127, 128, F * T * 2 * A2 * 2).astype(np.int8).reshape(F, T * 2, A2 * 2)
+    blk[:, :, 2 * a0:2 * (a0 + A)] = ftpa.reshape(F, T * 2, A * 2)
+    d_b = _dev(capi, blk.reshape(-1))
+    d_o2 = capi.DeviceBuffer(F * (T // ts) * B)
+    plan.beamform_subarray(d_b, A2, a0, d_w, d_o2)
+    np.testing.assert_array_equal(d_o2.download(), ref)
+
+
+@pytest.mark.parametrize("variant", [0, 5, 9, 10])
+def test_cb_set_levels_applies_to_later_blocks(gpu, oracle, variant):
+    """New levels on a live plan (PipelineConfig's output level changing
+    between blocks) apply to the next beamform call: the quantiser's
+    thresholds are rebuilt with them, including after a switch to levels the
+    thresholds do not cover (zero scale) and back."""
+    capi = gpu
+    F, T, A, B, ts = 8, 512, 16, 64, 16
+    rng = np.random.default_rng(91 + variant)
+    ftpa = rng.integers(-40, 41, F * T * 2 * A * 2).astype(np.int8)
+    w = rng.integers(-100, 101, F * B * A * 2).astype(np.int8)
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, 1)
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    d_o = capi.DeviceBuffer(F * (T // ts) * B)
+    off1, sc1 = _levels(oracle, A, ts, 1, F)
+    plan = capi.CbPlan(A, B, F, ts, 1, off1, sc1, nsamples_per_heap_out=T // ts)
+    plan.set_variant(variant)
+    levels = [(off1, sc1),
+              (off1 * 1.5, sc1 * 0.25),
+              (off1, np.where(np.arange(F) % 3 == 0, 0.0, sc1).astype(np.float32)),
+              (off1 * 0.5, sc1 * 3.0)]
+    for k, (off, sc) in enumerate(levels):
+        if k:
+            plan.set_levels(off, sc)
+        plan.beamform(d_v, d_w, d_o)
+        np.testing.assert_array_equal(d_o.download(), oracle.cb_tbtf(p, off, sc, T // ts), err_msg=f"levels {k}")

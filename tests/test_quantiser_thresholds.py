@@ -1,0 +1,78 @@
+"""The coherent beamformer's quantiser (cb_tr_k) replaces the division of the
+exact quantiser by a per-channel table: thr[j] is the least integer power
+that quantises to at least j - 128, and a quotient estimate
+RN((P - offset) * RN(1 / scale)) is corrected by the two thresholds around
+it.  This checks that arithmetic on the CPU (numpy float64 is IEEE double,
+as on the device) against the oracle's quantiser, with thresholds found by
+binary search on the oracle itself; the GPU tests check the kernel.
+"""
+import numpy as np
+import pytest
+
+INT64_MIN, INT64_MAX = np.iinfo(np.int64).min, np.iinfo(np.int64).max
+
+
+def _thresholds(oracle, off, sc, mode):
+    t = np.empty(257, np.int64)
+    t[0], t[256] = INT64_MIN, INT64_MAX
+    for j in range(1, 256):
+        k = j - 128
+        lo, hi = 0, 1 << 62
+        if oracle.quantise(float(hi), off, sc, mode) < k:
+            t[j] = INT64_MAX
+            continue
+        while lo < hi:
+            mid = lo + (hi - lo) // 2
+            if oracle.quantise(float(mid), off, sc, mode) >= k:
+                hi = mid
+            else:
+                lo = mid + 1
+        t[j] = lo
+    return t
+
+
+def _quantise_thr(P, off, sc, thr, mode):
+    y = (P.astype(np.float64) - np.float64(np.float32(off))) * (1.0 / np.float64(np.float32(sc)))
+    e = np.trunc(y) if mode else np.rint(y)
+    j = np.clip(e, -128, 127).astype(np.int64) + 128
+    j += (P >= thr[j + 1]).astype(np.int64) - (P < thr[j]).astype(np.int64)
+    return (j - 128).astype(np.int8)
+
+
+# (offset, scale): config-2 style levels, levels whose quotients are exact
+# integers and half-integers, large and tiny scales, a negative offset
+LEVELS = [(1.41e9, 2.6e7), (3.5, 1.0), (12.0, 8.0), (6.0e4, 2048.0), (7.0e12, 3.1e10),
+          (-2.5e5, 0.37), (0.0, 1e-3), (5.0e15, 1.0e13)]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("off,sc", LEVELS)
+def test_threshold_quantiser_matches_exact_division(oracle, off, sc, mode):
+    thr = _thresholds(oracle, off, sc, mode)
+    assert np.all(thr[1:] >= thr[:-1])
+    reach = thr[1:256][thr[1:256] < INT64_MAX]
+    # powers at and next to every threshold, and a spread over 2^0 .. 2^56
+    near = (reach[:, None] + np.arange(-3, 4)[None, :]).reshape(-1)
+    rng = np.random.default_rng(3)
+    spread = (rng.integers(0, 1 << 56, 20000) >> rng.integers(0, 56, 20000)).astype(np.int64)
+    P = np.concatenate([near[near >= 0], spread])
+    got = _quantise_thr(P, off, sc, thr, mode)
+    want = oracle.cb_tbtf(P.astype(np.float64).reshape(1, 1, -1), np.float32([off]), np.float32([sc]), 1,
+                          mode).reshape(-1)
+    np.testing.assert_array_equal(got, want)
+    # the levels really cross steps: most thresholds are reachable
+    assert reach.size > 100
+
+
+def test_threshold_estimate_alone_is_not_exact(oracle):
+    """Without the threshold correction the estimate disagrees with the
+    exact quantiser at some boundary (the correction is not idle)."""
+    off, sc, mode = 3.5, 1.0, 0
+    thr = _thresholds(oracle, off, sc, mode)
+    P = np.arange(0, 300, dtype=np.int64)
+    y = (P.astype(np.float64) - 3.5) / 1.0
+    est = np.clip(np.rint(y), -128, 127).astype(np.int8)   # half-even, not half-away
+    want = oracle.cb_tbtf(P.astype(np.float64).reshape(1, 1, -1), np.float32([off]), np.float32([sc]), 1,
+                          mode).reshape(-1)
+    assert np.any(est != want)
+    np.testing.assert_array_equal(_quantise_thr(P, off, sc, thr, mode), want)

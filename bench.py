@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--separate-ib", action="store_true",
                     help="run the incoherent beam as its own kernel instead of inside the split-transpose pass")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 40/16/8 subarray leg")
+    ap.add_argument("--no-corner-turns", action="store_true",
+                    help="skip the TUSE corner-turn and F-engine reorder leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the four-feed serving leg")
     ap.add_argument("--serving-seconds", type=float, default=8.0, help="Poisson arrival window of the serving leg")
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0, help="serving engine HBM budget")
@@ -218,6 +220,8 @@ def main():
     ib.close()
     for b in (d_taftp, d_ftpa, d_delays, d_w, d_cb, d_ib):
         b.free()
+    if not args.no_corner_turns:
+        result["corner_turns"] = corner_turn_leg(min(K, 100), args.warmup)
     if not args.no_batch:
         result["batched_subarrays"] = batch_leg(min(K, 100), args.warmup)
     if not args.no_serving:
@@ -261,6 +265,45 @@ def calibrate_copy():
     print(json.dumps({"calibrate_copy": {"kernel": "torch.neg int32", "read_bytes": 4 * n, "write_bytes": 4 * n,
                                          "launches": 3}}), flush=True)
     return 0
+
+
+def corner_turn_leg(K, warmup):
+    """The byte reorders either side of the beamformer, device-resident, HIP
+    events around K launches on one stream: the TUSE beam corner turn of 512
+    beams x 16 bands x 512 samples (4 groups of 128) x 64 channels (256 MiB),
+    and the F-engine heap reorder of 1024 timestamps x 256 channels x 256
+    samples x 2 pols (256 MiB).  HBM-bound: each byte read once and written
+    once, so the algorithmic traffic is twice the block."""
+    from psrdada_cpp_amd import capi, workloads as wl
+
+    out = {}
+    s = capi.Stream()
+    C, S, F, B, G = 64, 128, 16, 512, 4
+    n = G * B * F * S * C
+    d_in = capi.DeviceBuffer.from_array(capi.synth_gaussian_int8(7, 30.0, n), s)
+    d_out = capi.DeviceBuffer(n)
+    legs = [("tuse_corner_turn", lambda: capi.tuse_transpose(d_in, d_out, C, S, F, B, G, stream=s),
+             "512 beams x 16 bands x 512 samples (4 groups of 128) x 64 channels, per-beam [group][sample][band][chan]"),
+            ("feng_to_dada", lambda: capi.feng_to_dada(d_in, d_out, 256, stream=s),
+             "1024 timestamps of [256 channels][256 samples] x 2 pols char2 -> [256 samples][256 channels]")]
+    for name, fn, workload in legs:
+        for _ in range(warmup):
+            fn()
+        s.sync()
+        e0, e1 = capi.Event(), capi.Event()
+        e0.record(s)
+        for _ in range(K):
+            fn()
+        e1.record(s)
+        s.sync()
+        ms = e0.elapsed_ms(e1) / K
+        gbs = 2 * n / (ms * 1e-3) / 1e9
+        out[name] = {"workload": workload, "bytes_per_launch": 2 * n, "kernel_ms": round(ms, 4),
+                     "achieved_gbs": round(gbs, 1), "peak_gbs": wl.PEAK_HBM_GBS,
+                     "frac": round(gbs / wl.PEAK_HBM_GBS, 4)}
+    d_in.free()
+    d_out.free()
+    return out
 
 
 def batch_leg(K, warmup):

@@ -217,6 +217,29 @@ int fbf_split_transpose_ib(const void* taftp, size_t taftp_bytes, void* ftpa, si
                            int nchans, int nsamples_per_heap, fbf_ib_plan* ib_plan, void* tf,
                            size_t tf_bytes, fbf_stream_t stream);
 
+/* ---- corner turns either side of the beamformer ------------------------ */
+
+/* TUSE beam corner turn: replaces transpose::do_transpose
+ * (meerkat/tuse/transpose_to_dada.hpp:20, meerkat/tuse/src/transpose_to_dada.cpp:19-48)
+ * for the beams [beam0, beam0 + nbeams_out) at once.
+ *   in  [ngroups][nbeams][nfreq][nsamples][nchans] bytes (in_bytes may exceed it)
+ *   out [nbeams_out][ngroups][nsamples][nfreq][nchans] bytes: beam b's part,
+ *       out + (b - beam0) * ngroups * nsamples * nfreq * nchans, is what
+ *       do_transpose(..., beamnum = b, ...) writes.
+ * Device pointers; asynchronous on `stream`. */
+int fbf_tuse_transpose(const void* in, size_t in_bytes, void* out, size_t out_bytes, int nchans,
+                       int nsamples, int nfreq, int nbeams, int ngroups, int beam0, int nbeams_out,
+                       fbf_stream_t stream);
+
+/* F-engine heap reorder: replaces kernels::feng_heaps_to_dada
+ * (meerkat/tools/feng_to_dada.cuh:16-19, meerkat/tools/src/feng_to_dada.cu:54-80) as
+ * launched by FengToDada::operator() (meerkat/tools/detail/feng_to_dada.cu:27-52).
+ * Per timestamp, [nchans][256] 32-bit words (256 samples per heap x 2 pols x char2)
+ * -> [256][nchans].  bytes must be a whole number of timestamps; any nchans > 0
+ * (the reference kernel needs nchans % 32 == 0).  Out of place: in != out.
+ * Device pointers; asynchronous on `stream`. */
+int fbf_feng_to_dada(const void* in, void* out, size_t bytes, int nchans, fbf_stream_t stream);
+
 /* ---- delay -> weights (WeightsManager.cu:14-123) ----------------------- */
 typedef struct fbf_weights_plan fbf_weights_plan;
 

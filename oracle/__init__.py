@@ -21,6 +21,10 @@ Pinning (what checks each function beyond restating it):
                       integer cycles -> (127, 0), numpy FP64
   power_levels     -- closed-form formulas of PipelineConfig.cpp:109-127
   channel_freqs    -- closed form of PipelineConfig.cpp:180-195
+  tuse_transpose   -- hand-derived index KAT, numpy reshape/transpose,
+                      permutation invariant (per-beam byte histogram)
+  feng_to_dada     -- hand-derived index KAT, numpy transpose, involution
+                      with the inverse reorder
 """
 from __future__ import annotations
 
@@ -207,4 +211,29 @@ def channel_frequencies(cfreq, bw, nchans):
     out = np.empty(nchans, dtype=np.float32)
     _load().oracle_channel_frequencies(ctypes.c_float(cfreq), ctypes.c_float(bw),
                                        ctypes.c_int(nchans), _p(out))
+    return out
+
+
+def tuse_transpose(block, nchans, nsamples, nfreq, nbeams, ngroups, beam0=0, nbeams_out=None):
+    """TUSE corner turn, per beam [ngroups][nsamples][nfreq][nchans]
+    (meerkat/tuse/src/transpose_to_dada.cpp:19-48)."""
+    block = _i8(block)
+    if nbeams_out is None:
+        nbeams_out = nbeams - beam0
+    assert block.size >= ngroups * nbeams * nfreq * nsamples * nchans
+    out = np.empty(nbeams_out * ngroups * nsamples * nfreq * nchans, np.int8)
+    _load().oracle_tuse_transpose(_p(block), _p(out), ctypes.c_int(nchans), ctypes.c_int(nsamples),
+                                  ctypes.c_int(nfreq), ctypes.c_int(nbeams), ctypes.c_int(ngroups),
+                                  ctypes.c_int(beam0), ctypes.c_int(nbeams_out))
+    return out
+
+
+def feng_to_dada(words, nchans):
+    """F-engine heap reorder, per timestamp [nchans][256] -> [256][nchans]
+    32-bit words (meerkat/tools/src/feng_to_dada.cu:54-80)."""
+    words = np.ascontiguousarray(words, dtype=np.int32)
+    assert words.size % (nchans * 256) == 0
+    out = np.empty_like(words)
+    _load().oracle_feng_to_dada(_p(words), _p(out), ctypes.c_int(nchans),
+                                ctypes.c_long(words.size // (nchans * 256)))
     return out

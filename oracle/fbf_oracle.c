@@ -402,3 +402,47 @@ void oracle_channel_frequencies(float cfreq, float bw, int nchans, float* freqs)
         freqs[c] = y + x;
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* TUSE corner turn: meerkat/tuse/src/transpose_to_dada.cpp:19-48       */
+/* ------------------------------------------------------------------ */
+void oracle_tuse_transpose(const int8_t* in, int8_t* out, int nchans,
+                           int nsamples, int nfreq, int nbeams, int ngroups,
+                           int beam0, int nbeams_out)
+{
+    /* The reference copies one beam of one group (nfreq bands of
+     * nsamples x nchans) and writes, for every sample and band, the band's
+     * nchans bytes at isamp * nchans * nfreq + iband * nchans. */
+    const size_t per_beam = (size_t)nchans * nsamples * nfreq;  /* skipbeam / beamnum */
+    const size_t per_group = per_beam * nbeams;                  /* skipgroup */
+    const size_t skipband = (size_t)nchans * nsamples;
+    const size_t skipallchans = (size_t)nchans * nfreq;
+    for (int bo = 0; bo < nbeams_out; ++bo) {
+        const int8_t* inb = in + (size_t)(beam0 + bo) * per_beam;
+        int8_t* outb = out + (size_t)bo * per_beam * ngroups;
+        for (int g = 0; g < ngroups; ++g)
+            for (int s = 0; s < nsamples; ++s)
+                for (int f = 0; f < nfreq; ++f)
+                    memcpy(outb + (size_t)g * per_beam + (size_t)s * skipallchans + (size_t)f * nchans,
+                           inb + (size_t)g * per_group + (size_t)f * skipband + (size_t)s * nchans,
+                           (size_t)nchans);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* F-engine heaps: meerkat/tools/src/feng_to_dada.cu:54-80              */
+/* ------------------------------------------------------------------ */
+void oracle_feng_to_dada(const int32_t* in, int32_t* out, int nchans,
+                         long ntimestamps)
+{
+    /* MEERKAT_FENG_NSAMPS_PER_HEAP = 256 (meerkat/constants.hpp:4); the
+     * kernel's input index (chan * 256 + time) and output index
+     * (time * nchans + chan) within each timestamp's block. */
+    const long nsamps = 256;
+    for (long t = 0; t < ntimestamps; ++t) {
+        const int32_t* src = in + t * nchans * nsamps;
+        int32_t* dst = out + t * nchans * nsamps;
+        for (long s = 0; s < nsamps; ++s)
+            for (long c = 0; c < nchans; ++c) dst[s * nchans + c] = src[c * nsamps + s];
+    }
+}

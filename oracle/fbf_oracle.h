@@ -119,6 +119,20 @@ void oracle_power_levels(float input_level, float output_level,
 void oracle_channel_frequencies(float cfreq, float bw, int nchans,
                                 float* freqs);
 
+/* TUSE beam corner turn (meerkat/tuse/src/transpose_to_dada.cpp:19-48),
+ * do_transpose for each beam in [beam0, beam0 + nbeams_out):
+ * in  [ngroups][nbeams][nfreq][nsamples][nchans] bytes,
+ * out [nbeams_out][ngroups][nsamples][nfreq][nchans] bytes. */
+void oracle_tuse_transpose(const int8_t* in, int8_t* out, int nchans,
+                           int nsamples, int nfreq, int nbeams, int ngroups,
+                           int beam0, int nbeams_out);
+
+/* F-engine heaps to DADA order (meerkat/tools/src/feng_to_dada.cu:54-80):
+ * per timestamp [nchans][256] 32-bit words (two char2 polarisations) ->
+ * [256][nchans]. */
+void oracle_feng_to_dada(const int32_t* in, int32_t* out, int nchans,
+                         long ntimestamps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -123,6 +123,8 @@ _SIGS = {
     "fbf_batch_run": [vp, vp, sz, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), i32, vp],
     "fbf_cb_beamform_ws": [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp],
     "fbf_split_transpose_ib": [vp, sz, vp, sz, i32, i32, vp, vp, sz, vp],
+    "fbf_tuse_transpose": [vp, sz, vp, sz, i32, i32, i32, i32, i32, i32, i32, vp],
+    "fbf_feng_to_dada": [vp, vp, sz, i32, vp],
     "fbf_cb_plan_reserve": [vp, C.c_longlong],
     "fbf_server_create": [i32, sz, pvp],
     "fbf_server_destroy": [vp],
@@ -352,6 +354,22 @@ def split_transpose_ib(taftp, ftpa, nchans, ib_plan, tf, nsamples_per_heap=256, 
     """Full-antenna split-transpose and incoherent beam in one pass."""
     call("fbf_split_transpose_ib", taftp.ptr, taftp.nbytes, ftpa.ptr, ftpa.nbytes, nchans, nsamples_per_heap,
          ib_plan.handle, tf.ptr, tf.nbytes, _s(stream))
+
+
+def tuse_transpose(block, out, nchans, nsamples, nfreq, nbeams, ngroups, beam0=0, nbeams_out=None, stream=None):
+    """TUSE beam corner turn of the beams [beam0, beam0 + nbeams_out):
+    [ngroups][nbeams][nfreq][nsamples][nchans] -> per beam
+    [ngroups][nsamples][nfreq][nchans] (device buffers)."""
+    if nbeams_out is None:
+        nbeams_out = nbeams - beam0
+    call("fbf_tuse_transpose", block.ptr, block.nbytes, out.ptr, out.nbytes, nchans, nsamples, nfreq, nbeams,
+         ngroups, beam0, nbeams_out, _s(stream))
+
+
+def feng_to_dada(block, out, nchans, stream=None):
+    """F-engine heap reorder, per timestamp [nchans][256] -> [256][nchans]
+    32-bit words (device buffers of equal size)."""
+    call("fbf_feng_to_dada", block.ptr, out.ptr, block.nbytes, nchans, _s(stream))
 
 
 class CbPlan:

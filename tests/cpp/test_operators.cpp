@@ -28,6 +28,8 @@
 #include "psrdada_cpp/meerkat/fbfuse/PipelineConfig.hpp"
 #include "psrdada_cpp/meerkat/fbfuse/SplitTranspose.hpp"
 #include "psrdada_cpp/meerkat/fbfuse/WeightsManager.hpp"
+#include "psrdada_cpp/meerkat/tools/feng_to_dada.hpp"
+#include "psrdada_cpp/meerkat/tuse/transpose_to_dada.hpp"
 #include "psrdada_cpp/raw_bytes.hpp"
 
 using namespace psrdada_cpp;
@@ -542,6 +544,105 @@ void pipeline_outputs_match_operators()
     fbf_stream_destroy(stream);
 }
 
+// A DADA-handler stand-in that keeps what it was given.
+struct BytesSink {
+    std::vector<char> header;
+    std::vector<char> data;
+    int calls = 0;
+    void init(RawBytes& block) { header.assign(block.ptr(), block.ptr() + block.used_bytes()); }
+    bool operator()(RawBytes& block)
+    {
+        data.assign(block.ptr(), block.ptr() + block.used_bytes());
+        ++calls;
+        return false;
+    }
+};
+
+// TransposeToDada (meerkat/tuse/transpose_to_dada.hpp) with the reference's
+// handler-per-beam use (transpose_to_dada_cli.cpp:99-109): every beam's
+// buffer equals the oracle's do_transpose for that beam; do_transpose itself
+// on host and on device buffers; the reference's default geometry.
+void tuse_transpose_to_dada()
+{
+    const int nbeams = 6, nchans = 64, nsamples = 32, nfreq = 8, ngroups = 3;
+    std::vector<std::shared_ptr<BytesSink>> sinks;
+    for (int i = 0; i < nbeams; ++i) sinks.push_back(std::make_shared<BytesSink>());
+    meerkat::tuse::TransposeToDada<BytesSink> tr(nbeams, sinks);
+    EXPECT(tr.nchans() == 128 && tr.nsamples() == 64 && tr.nfreq() == 32 && tr.ngroups() == 10 && tr.nbeams() == 6);
+    tr.set_nchans(nchans);
+    tr.set_nsamples(nsamples);
+    tr.set_nfreq(nfreq);
+    tr.set_ngroups(ngroups);
+    const std::size_t per_beam = (std::size_t)nchans * nsamples * nfreq * ngroups;
+    std::vector<char2> v = gaussian_char2(4242, 40.0, per_beam * nbeams / 2, false);
+    char* blk = reinterpret_cast<char*>(v.data());
+    std::vector<char> hdr(4096, 'h');
+    RawBytes hrb(hdr.data(), hdr.size(), hdr.size());
+    tr.init(hrb);
+    RawBytes block(blk, per_beam * nbeams, per_beam * nbeams);
+    EXPECT(tr(block) == false);
+    std::vector<int8_t> expect(per_beam);
+    for (int b = 0; b < nbeams; ++b) {
+        oracle_tuse_transpose(reinterpret_cast<const int8_t*>(blk), expect.data(), nchans, nsamples, nfreq, nbeams,
+                              ngroups, b, 1);
+        EXPECT(sinks[b]->calls == 1 && sinks[b]->header.size() == 4096);
+        EXPECT(sinks[b]->data.size() == per_beam);
+        EXPECT(std::memcmp(sinks[b]->data.data(), expect.data(), per_beam) == 0);
+    }
+    // do_transpose for one beam, host buffers and device buffers
+    std::vector<char> one(per_beam);
+    RawBytes out_rb(one.data(), per_beam);
+    meerkat::tuse::transpose::do_transpose(out_rb, block, nchans, nsamples, nfreq, 4, nbeams, ngroups);
+    oracle_tuse_transpose(reinterpret_cast<const int8_t*>(blk), expect.data(), nchans, nsamples, nfreq, nbeams,
+                          ngroups, 4, 1);
+    EXPECT(std::memcmp(one.data(), expect.data(), per_beam) == 0);
+    DeviceVector<char> din(std::vector<char>(blk, blk + per_beam * nbeams)), dout(per_beam);
+    RawBytes din_rb(din.data(), din.bytes(), din.bytes(), true), dout_rb(dout.data(), dout.bytes(), 0, true);
+    meerkat::tuse::transpose::do_transpose(dout_rb, din_rb, nchans, nsamples, nfreq, 4, nbeams, ngroups);
+    std::vector<char> back = dout.to_host();
+    EXPECT(std::memcmp(back.data(), expect.data(), per_beam) == 0);
+    // a block too small for the geometry throws
+    bool threw = false;
+    RawBytes small(blk, per_beam, per_beam);
+    try {
+        tr(small);
+    } catch (std::runtime_error const&) {
+        threw = true;
+    }
+    EXPECT(threw);
+}
+
+// FengToDada (meerkat/tools/feng_to_dada.cuh): the block is reordered in
+// place to the oracle's order before the handler sees it; a block that is
+// not a whole number of timestamps throws the reference's error.
+void feng_to_dada_reorder()
+{
+    const std::size_t nchans = 96, nts = 3;
+    const std::size_t bytes = nchans * 256 * 2 * sizeof(char2) * nts;
+    std::vector<char2> v = gaussian_char2(777, 50.0, bytes / 2, false);
+    std::vector<int32_t> words(bytes / 4), expect(bytes / 4);
+    std::memcpy(words.data(), v.data(), bytes);
+    oracle_feng_to_dada(words.data(), expect.data(), (int)nchans, (long)nts);
+    BytesSink sink;
+    meerkat::tools::FengToDada<BytesSink> f2d(nchans, sink);
+    std::vector<char> hdr(4096, 'h');
+    RawBytes hrb(hdr.data(), hdr.size(), hdr.size());
+    f2d.init(hrb);
+    RawBytes block(reinterpret_cast<char*>(words.data()), bytes, bytes);
+    EXPECT(f2d(block) == false);
+    EXPECT(sink.calls == 1 && sink.header.size() == 4096);
+    EXPECT(std::memcmp(words.data(), expect.data(), bytes) == 0);
+    EXPECT(std::memcmp(sink.data.data(), expect.data(), bytes) == 0);
+    bool threw = false;
+    RawBytes ragged(reinterpret_cast<char*>(words.data()), bytes, bytes - 4);
+    try {
+        f2d(ragged);
+    } catch (std::runtime_error const&) {
+        threw = true;
+    }
+    EXPECT(threw && sink.calls == 1);
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -565,6 +666,8 @@ int main(int argc, char** argv)
         {"DelayManagerTester.test_bad_keys", delay_manager_bad_keys},
         {"PipelineTester.simple_run_test", pipeline_simple_run},
         {"Pipeline.outputs_match_operators", pipeline_outputs_match_operators},
+        {"TransposeToDada.beams_match_oracle", tuse_transpose_to_dada},
+        {"FengToDada.reorders_in_place", feng_to_dada_reorder},
     };
     int ran = 0;
     for (auto& c : cases) {

@@ -356,3 +356,47 @@ def test_golden_fixtures(oracle):
     np.testing.assert_array_equal(s.reshape(-1), np.array(g["sums"], np.int64))
     p = oracle.cb_power(ftpa, w, F, T, A, B, ts, fs)
     np.testing.assert_array_equal(p.reshape(-1), np.array(g["power"], np.float64))
+
+
+# ---------------------------------------------------------------------------
+# TUSE corner turn (meerkat/tuse/src/transpose_to_dada.cpp:19-48) and the
+# F-engine heap reorder (meerkat/tools/src/feng_to_dada.cu:54-80)
+# ---------------------------------------------------------------------------
+def test_tuse_transpose_hand_kat(oracle):
+    # 2 groups, 2 beams, 2 bands, 3 samples, 2 channels; byte i = i.
+    C, S, F, B, G = 2, 3, 2, 2, 2
+    blk = np.arange(G * B * F * S * C, dtype=np.int64).astype(np.int8)
+    out = oracle.tuse_transpose(blk, C, S, F, B, G)
+    # beam 1, group 1, sample 2, band 1, channel 0: the reference reads
+    # skipbeam + igroup * skipgroup + iband * skipband + isamp * nchans
+    # = 1*12 + 1*24 + 1*6 + 2*2 = 46 and writes it at
+    # beam 1's buffer + igroup * 12 + isamp * nchans * nfreq + iband * nchans = 12 + 8 + 2 = 22.
+    assert out[1 * 24 + 22] == 46
+    # beam 0, group 0, sample 1, band 0, channel 1: input 0 + 0 + 0 + 2 + 1 = 3 at output 4 + 0 + 1 = 5
+    assert out[5] == 3
+
+
+@pytest.mark.parametrize("C,S,F,B,G,b0,nb", [(64, 16, 8, 5, 3, 0, 5), (3, 7, 5, 4, 2, 1, 2), (128, 64, 32, 2, 10, 1, 1)])
+def test_tuse_transpose_matches_numpy(oracle, C, S, F, B, G, b0, nb):
+    rng_ = np.random.default_rng(C + S)
+    blk = rng_.integers(-128, 128, G * B * F * S * C).astype(np.int8)
+    out = oracle.tuse_transpose(blk, C, S, F, B, G, b0, nb)
+    ref = blk.reshape(G, B, F, S, C).transpose(1, 0, 3, 2, 4)[b0:b0 + nb]
+    np.testing.assert_array_equal(out, ref.reshape(-1))
+    # a permutation of each beam's bytes
+    for k in range(nb):
+        a = np.sort(out.reshape(nb, -1)[k])
+        b = np.sort(blk.reshape(G, B, -1)[:, b0 + k].reshape(-1))
+        np.testing.assert_array_equal(a, b)
+
+
+def test_feng_to_dada_hand_kat_and_numpy(oracle):
+    nchans, T = 32, 3
+    w = np.arange(T * nchans * 256, dtype=np.int64).astype(np.int32)
+    out = oracle.feng_to_dada(w, nchans)
+    # timestamp 2, sample 5, channel 7 comes from input word 2*8192 + 7*256 + 5
+    assert out[2 * 8192 + 5 * nchans + 7] == 2 * 8192 + 7 * 256 + 5
+    np.testing.assert_array_equal(out, w.reshape(T, nchans, 256).transpose(0, 2, 1).reshape(-1))
+    # the inverse reorder ([256][nchans] -> [nchans][256]) brings it back
+    back = out.reshape(T, 256, nchans).transpose(0, 2, 1).reshape(-1)
+    np.testing.assert_array_equal(back, w)

@@ -1064,6 +1064,10 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
         for (int g = 0; g < NG; ++g)
 #pragma unroll
             for (int sl = 0; sl < NSLOT; ++sl) pw[g][sl] = 0;
+        // Passes past the end of the channel (the last slots of a one-round
+        // grid) still take part in the staging and the barriers but leave
+        // the MFMA pipe to the other waves of the SIMD.
+        const bool live = wave_active && tile0 + pass * ta.ntp < ta.ntiles;
 
         // Tiles of the pass: NSLOT/SPT groups of tps tiles (tps = 1 unless
         // a window spans several tiles).
@@ -1083,7 +1087,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
                         is = is + 1 == NB ? 0 : is + 1;
                     }
                 }
-                if (wave_active) {
+                if (live) {
                     const int8_t* src = lds + rs * St::SET;
                     v4i a[KSTEPS];
 #pragma unroll

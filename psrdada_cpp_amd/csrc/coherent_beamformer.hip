@@ -896,7 +896,8 @@ struct TrStage {
 
 template <int RT, int NSLOT>
 struct TrRed {
-    static constexpr int VALS = RT == 2 ? NSLOT / 2 : NSLOT;   // totals per lane after the reduce-scatter
+    // totals per lane after the reduce-scatter
+    static constexpr int VALS = RT == 2 ? NSLOT / 2 : (RT / 4) * NSLOT;
 };
 
 // RT: beam tiles per wave (RT / 2 groups of 32 beams, a Re and an Im tile
@@ -918,8 +919,8 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     constexpr int VALS = TrRed<RT, NSLOT>::VALS;
     constexpr int NB = St::NB;
     constexpr int NG = RT / 2;                     // groups of 32 beams per wave
-    static_assert(RT == 2 || RT == 4, "RT");
-    static_assert(NSLOT % SPT == 0 && (RT == 4 || NSLOT % 2 == 0), "slots");
+    static_assert(RT == 2 || RT == 4 || RT == 8, "RT");
+    static_assert(NSLOT % SPT == 0 && (RT != 2 || NSLOT % 2 == 0), "slots");
     static_assert(St::M <= 2, "at most two staging loads per wave per set");
     extern __shared__ __align__(16) int8_t lds[];
     const CbArgs& p = ta.p;
@@ -1045,17 +1046,19 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     if (tid < kThr) thr[tid] = thr_v;
     // Quantise and store one pass's totals (staged [fo][to][b] int8); with
     // FS > 1 the waves of a beam group share the totals out.
-    const int b_out = RT == 2 ? beam_base + c : beam_base + 32 * h + c;
-    int8_t* const out_f = static_cast<int8_t*>(p.out) + (size_t)fo * p.nsamples_out * p.B + b_out;
+    // Value v of a lane: slot 2v + h of beam c (RT = 2), else slot
+    // v % NSLOT of beam 64 (v / NSLOT) + 32 h + c.
+    int8_t* const out_f = static_cast<int8_t*>(p.out) + (size_t)fo * p.nsamples_out * p.B;
     auto store_pass = [&](const long long (&t)[VALS], int w0) {
 #pragma unroll
         for (int v = 0; v < VALS; ++v) {
             if (FS > 1 && v % FS != wc) continue;
-            const int sl = RT == 2 ? 2 * v + h : v;
+            const int sl = RT == 2 ? 2 * v + h : v % NSLOT;
+            const int b_out = RT == 2 ? beam_base + c : beam_base + 64 * (v / NSLOT) + 32 * h + c;
             const long long to = (long long)w0 + sl;
             if (!wave_active || b_out >= p.B || to >= p.nsamples_out) continue;
-            out_f[(size_t)to * p.B] = q_thr ? quantise_thr(t[v], qz, thr, p.quant)
-                                            : quantise((double)t[v], q_off, q_sc, p.quant);
+            out_f[(size_t)to * p.B + b_out] = q_thr ? quantise_thr(t[v], qz, thr, p.quant)
+                                                    : quantise((double)t[v], q_off, q_sc, p.quant);
         }
     };
     for (int pass = 0; pass < ta.npasses; ++pass) {
@@ -1101,14 +1104,18 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 
         // A window's power is split over the row halves (lane and lane ^ 32):
         // one v_permlane32_swap reduce-scatter per value, lanes below 32
-        // keeping beam group 0 (RT = 4) or the even slots (RT = 2).
+        // keeping the even beam group of each pair (RT >= 4) or the even
+        // slots (RT = 2).
         long long tot[VALS];
         if constexpr (RT == 2) {
 #pragma unroll
             for (int v = 0; v < VALS; ++v) tot[v] = swap32_sum(pw[0][2 * v], pw[0][2 * v + 1]);
         } else {
 #pragma unroll
-            for (int sl = 0; sl < NSLOT; ++sl) tot[sl] = swap32_sum(pw[0][sl], pw[1][sl]);
+            for (int gp = 0; gp < NG / 2; ++gp)
+#pragma unroll
+                for (int sl = 0; sl < NSLOT; ++sl)
+                    tot[gp * NSLOT + sl] = swap32_sum(pw[2 * gp][sl], pw[2 * gp + 1][sl]);
         }
 
         // Sum over the fscrunch channels (one per wave of a beam group).
@@ -1147,13 +1154,14 @@ inline int cb_tr_nslot(int rt, int W)
 
 inline int cb_tr_nb(int ni) { return 60 / ni < 2 ? 2 : (60 / ni > 8 ? 8 : 60 / ni); }
 
-// Time slots per work-group of the transposed kernel.
-constexpr int kTrNts = 3;
+// Time slots per work-group of the transposed kernel: three waves per SIMD,
+// or two with 128 beams per wave (RT = 8: 256 VGPRs a wave).
+constexpr int tr_nts(int rt) { return rt == 8 ? 2 : 3; }
 
-inline size_t cb_tr_shm(int fs, int ksteps, int vals)
+inline size_t cb_tr_shm(int fs, int ksteps, int vals, int nts)
 {
-    const int nw = 4 * kTrNts;
-    const int ni = kTrNts * fs * ksteps;
+    const int nw = 4 * nts;
+    const int ni = nts * fs * ksteps;
     return (size_t)cb_tr_nb(ni) * ni * 1024 + ((ni % nw) ? (size_t)nw * 1024 : 0) +
            (fs > 1 ? (size_t)nw * vals * 64 * sizeof(long long) : 0) + kThr * sizeof(long long);
 }
@@ -1164,11 +1172,11 @@ inline bool cb_tr_supported(int A, int ts, int fs, int rt)
     if (ts < 4 || (ts & (ts - 1)) != 0) return false;
     if (!(fs == 1 || fs == 2 || fs == 4)) return false;
     const int ksteps = (2 * A + 31) / 32;
-    if (rt * ksteps > 16) return false;
+    if (rt == 8 ? (fs != 1 || ksteps > 4) : rt * ksteps > 16) return false;
     if (fs * ksteps > 8) return false;
     const int nslot = cb_tr_nslot(rt, 2 * ts);
-    const int vals = rt == 2 ? nslot / 2 : nslot;
-    return cb_tr_shm(fs, ksteps, vals) <= 65536;
+    const int vals = rt == 2 ? nslot / 2 : (rt / 4) * nslot;
+    return cb_tr_shm(fs, ksteps, vals, tr_nts(rt)) <= 65536;
 }
 
 // Compute units of the current device (cached per device).
@@ -1193,8 +1201,8 @@ inline int device_cu_count()
 template <int RT, int KSTEPS, int SPT, int NSLOT, int FS>
 hipError_t launch_tr_k(CbTrArgs a, int passes, hipStream_t s)
 {
-    constexpr int NTS = kTrNts;
-    const size_t shm = cb_tr_shm(FS, KSTEPS, TrRed<RT, NSLOT>::VALS);
+    constexpr int NTS = tr_nts(RT);
+    const size_t shm = cb_tr_shm(FS, KSTEPS, TrRed<RT, NSLOT>::VALS, NTS);
     static int occ = 0;
     if (occ == 0) {
         int o = 0;
@@ -1257,9 +1265,14 @@ hipError_t launch_tr_w(const CbTrArgs& a, int W, int ksteps, int passes, hipStre
 template <int RT>
 hipError_t launch_tr(const CbTrArgs& a, int fs, int W, int ksteps, int passes, hipStream_t s)
 {
-    if (fs == 1) return launch_tr_w<RT, 1>(a, W, ksteps, passes, s);
-    if (fs == 2) return launch_tr_w<RT, 2>(a, W, ksteps, passes, s);
-    return launch_tr_w<RT, 4>(a, W, ksteps, passes, s);
+    if constexpr (RT == 8) {
+        if (fs != 1 || ksteps > 4) return hipErrorInvalidValue;
+        return launch_tr_w<RT, 1>(a, W, ksteps, passes, s);
+    } else {
+        if (fs == 1) return launch_tr_w<RT, 1>(a, W, ksteps, passes, s);
+        if (fs == 2) return launch_tr_w<RT, 2>(a, W, ksteps, passes, s);
+        return launch_tr_w<RT, 4>(a, W, ksteps, passes, s);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1632,7 +1645,44 @@ __global__ __launch_bounds__(256) void tbtf_k(TbtfArgs a)
     const int bv = min(kTrBeams, a.B - b0), fv = min(a.ft, a.Fo - f0), tv = min(a.tt, a.ohs - t0);
     const long long tg0 = heap * a.ohs + t0;
 
-    if ((fv & 3) == 0 && (f0 & 3) == 0) {
+    // 16-byte path: beams, channels, B and Fo multiples of 16 (the tile's
+    // beams and channels then start 16-aligned, and so do the rows).
+    const bool wide = ((bv | fv | a.B | a.Fo) & 15) == 0;
+    if (wide) {
+        // Load item: 16 beams x 4 channels as four 16-byte loads (beam
+        // sixteens fastest, so a wave reads whole 128-byte rows); four 4 x 4
+        // byte transposes with v_perm_b32; sixteen words of 4 channels to
+        // LDS.
+        const int nb16 = bv >> 4, nfg = fv >> 2;
+        const int nitems = nb16 * tv * nfg;
+        const size_t fstep = (size_t)a.To * a.B;
+        for (int i = threadIdx.x; i < nitems; i += 256) {
+            const int bs = i % nb16;
+            const int rest = i / nb16;
+            const int t = rest % tv;
+            const int fg = rest / tv;
+            const int8_t* src = a.src + ((size_t)(f0 + 4 * fg) * a.To + tg0 + t) * a.B + b0 + 16 * bs;
+            const uint4 q0 = *reinterpret_cast<const uint4*>(src);
+            const uint4 q1 = *reinterpret_cast<const uint4*>(src + fstep);
+            const uint4 q2 = *reinterpret_cast<const uint4*>(src + 2 * fstep);
+            const uint4 q3 = *reinterpret_cast<const uint4*>(src + 3 * fstep);
+            uint32_t* l = reinterpret_cast<uint32_t*>(tr_lds) + (size_t)t * a.tp + (size_t)(16 * bs) * a.bp + fg;
+            auto quad = [&](uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3, uint32_t* lq) {
+                const uint32_t t0 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);
+                const uint32_t t1 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);
+                const uint32_t t2 = __builtin_amdgcn_perm(r3, r2, 0x05010400u);
+                const uint32_t t3 = __builtin_amdgcn_perm(r3, r2, 0x07030602u);
+                lq[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+                lq[a.bp] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+                lq[2 * a.bp] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+                lq[3 * a.bp] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+            };
+            quad(q0.x, q1.x, q2.x, q3.x, l);
+            quad(q0.y, q1.y, q2.y, q3.y, l + 4 * a.bp);
+            quad(q0.z, q1.z, q2.z, q3.z, l + 8 * a.bp);
+            quad(q0.w, q1.w, q2.w, q3.w, l + 12 * a.bp);
+        }
+    } else if ((fv & 3) == 0 && (f0 & 3) == 0) {
         // Load 4 channels x 4 beams as four 4-byte words (beam quads
         // fastest, so a wave reads whole 128-byte rows), transpose the 4 x 4
         // bytes with v_perm_b32 and write four words of 4 channels.
@@ -1685,7 +1735,26 @@ __global__ __launch_bounds__(256) void tbtf_k(TbtfArgs a)
     // Fo channels are contiguous in TBTF when the tile holds every channel.
     int8_t* out = a.dst + (((size_t)heap * a.B + b0) * a.ohs + t0) * a.Fo + f0;
     const size_t brow = (size_t)a.ohs * a.Fo;
-    if ((a.Fo & 3) == 0) {
+    if (wide) {
+        // Store item: 16 channels of one (beam, sample) as one 16-byte store
+        // (channel sixteens fastest, then samples: a beam's tv rows are one
+        // contiguous run).
+        const int n16 = fv >> 4;
+        const int nstore = bv * tv * n16;
+        for (int i = threadIdx.x; i < nstore; i += 256) {
+            const int w = i % n16;
+            const int rest = i / n16;
+            const int t = rest % tv;
+            const int b = rest / tv;
+            const uint32_t* l = reinterpret_cast<const uint32_t*>(tr_lds) + (size_t)t * a.tp + (size_t)b * a.bp + 4 * w;
+            uint4 v;
+            v.x = l[0];
+            v.y = l[1];
+            v.z = l[2];
+            v.w = l[3];
+            *reinterpret_cast<uint4*>(out + b * brow + (size_t)t * a.Fo + 16 * w) = v;
+        }
+    } else if ((a.Fo & 3) == 0) {
         const int nw = fv >> 2;
         const int nstore = bv * tv * nw;
         for (int i = threadIdx.x; i < nstore; i += 256) {
@@ -1896,6 +1965,7 @@ static const CbVariant kCbVariants[] = {
     {32, 2, true, true, false},    // transposed: 32 beams per wave
     {32, 4, true, true, false},    // transposed: 64 beams per wave
     {32, 4, true, true, true},     // transposed, wave-private staging: 64 beams per wave
+    {32, 8, true, true, false},    // transposed: 128 beams per wave, two waves per SIMD
 };
 constexpr int kCbVariantLds = 5;
 constexpr int kCbVariantTr2 = 8;
@@ -2086,8 +2156,9 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
         const int beams_per_wg = 16 * rt * (4 / d.fscrunch);
         ta.n_bgroups = (d.nbeams + beams_per_wg - 1) / beams_per_wg;
         const int ksteps = (2 * d.nantennas + 31) / 32;
-        hipError_t e = rt == 2 ? launch_tr<2>(ta, d.fscrunch, W, ksteps, passes, s)
-                               : launch_tr<4>(ta, d.fscrunch, W, ksteps, passes, s);
+        hipError_t e = rt == 2   ? launch_tr<2>(ta, d.fscrunch, W, ksteps, passes, s)
+                       : rt == 4 ? launch_tr<4>(ta, d.fscrunch, W, ksteps, passes, s)
+                                 : launch_tr<8>(ta, d.fscrunch, W, ksteps, passes, s);
         if (e != hipSuccess) return hip_status(e, "cb_tr_k launch");
         e = launch_tbtf(tbtf_args(static_cast<const int8_t*>(kout), static_cast<int8_t*>(out), To, d.nbeams, Fo,
                                   plan->ohs),

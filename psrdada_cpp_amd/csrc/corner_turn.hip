@@ -84,6 +84,66 @@ __global__ __launch_bounds__(256) void corner_turn_k(CtArgs a)
     }
 }
 
+// 4-byte elements, full 64 x 64 tiles: 16-byte global loads (4 words of an
+// input row) and stores (4 words of an output row); the tile stays in
+// 4-byte LDS words with a 65-word pitch, which keeps both the row writes
+// and the column reads free of bank conflicts.  Edge tiles take the general
+// path of corner_turn_k.
+__global__ __launch_bounds__(256) void corner_turn_w4_k(CtArgs a)
+{
+    extern __shared__ __align__(16) uint8_t ct_lds[];
+    uint32_t* tile = reinterpret_cast<uint32_t*>(ct_lds);
+    constexpr int T = 64, P = 65;
+    long long bid = blockIdx.x;
+    const int tc = (int)(bid % a.tiles_c);
+    bid /= a.tiles_c;
+    const int tr = (int)(bid % a.tiles_r);
+    bid /= a.tiles_r;
+    const long long i2 = bid % a.n2;
+    const long long i1 = bid / a.n2;
+    const uint32_t* in = reinterpret_cast<const uint32_t*>(a.in + i1 * a.in_s1 + i2 * a.in_s2);
+    uint32_t* out = reinterpret_cast<uint32_t*>(a.out + i1 * a.out_s1 + i2 * a.out_s2);
+    const int r0 = tr * T, c0 = tc * T;
+    const int nr = min(T, a.R - r0), nc = min(T, a.C - c0);
+    const uint32_t* src = in + (size_t)r0 * a.C + c0;
+    uint32_t* dst = out + (size_t)c0 * a.R + r0;
+    if (nr == T && nc == T) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            const int r = i >> 4, c = (i & 15) << 2;
+            const uint4 v = *reinterpret_cast<const uint4*>(src + (size_t)r * a.C + c);
+            uint32_t* l = tile + r * P + c;
+            l[0] = v.x;
+            l[1] = v.y;
+            l[2] = v.z;
+            l[3] = v.w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            const int c = i >> 4, r = (i & 15) << 2;
+            uint4 v;
+            v.x = tile[r * P + c];
+            v.y = tile[(r + 1) * P + c];
+            v.z = tile[(r + 2) * P + c];
+            v.w = tile[(r + 3) * P + c];
+            *reinterpret_cast<uint4*>(dst + (size_t)c * a.R + r) = v;
+        }
+    } else {
+        for (int i = threadIdx.x; i < nr * nc; i += 256) {
+            const int r = i / nc, c = i - r * nc;
+            tile[r * P + c] = src[(size_t)r * a.C + c];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nr * nc; i += 256) {
+            const int c = i / nr, r = i - c * nr;
+            dst[(size_t)c * a.R + r] = tile[r * P + c];
+        }
+    }
+}
+
 inline int ilog2_exact(long long v)
 {
     if (v <= 0 || (v & (v - 1)) != 0) return -1;
@@ -131,6 +191,19 @@ int ct_launch(CtArgs a, long long n1, int E, hipStream_t s)
 {
     const uintptr_t al = reinterpret_cast<uintptr_t>(a.in) | reinterpret_cast<uintptr_t>(a.out) |
                          (uintptr_t)a.in_s1 | (uintptr_t)a.in_s2 | (uintptr_t)a.out_s1 | (uintptr_t)a.out_s2;
+    if (E == 4 && ((al | (uintptr_t)a.R * 4 | (uintptr_t)a.C * 4) & 15) == 0) {
+        // word transposes with 16-byte global accesses (rows 16-byte aligned)
+        a.EU = 1;
+        a.T = 64;
+        a.tiles_r = (a.R + 63) / 64;
+        a.tiles_c = (a.C + 63) / 64;
+        const long long nblocks = n1 * a.n2 * a.tiles_r * a.tiles_c;
+        if (nblocks == 0) return FBF_OK;
+        FBF_REQUIRE(nblocks < INT_MAX, FBF_ERR_UNSUPPORTED, "corner turn: too many tiles for one launch");
+        corner_turn_w4_k<<<dim3((unsigned)nblocks), 256, 64 * 65 * 4, s>>>(a);
+        FBF_HIP_TRY(hipGetLastError());
+        return FBF_OK;
+    }
     if (E % 16 == 0 && (al & 15) == 0) return ct_launch_u<uint4>(a, n1, E, s);
     if (E % 4 == 0 && (al & 3) == 0) return ct_launch_u<uint32_t>(a, n1, E, s);
     return ct_launch_u<uint8_t>(a, n1, E, s);

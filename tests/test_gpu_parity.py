@@ -325,7 +325,7 @@ def test_weights_fp64_phase_config4(gpu, oracle, epoch):
     assert (diff > 0).mean() < 1e-3
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11])
 @pytest.mark.parametrize("F,T,A,B,ts,fs", [(4, 512, 64, 512, 16, 1), (4, 256, 44, 96, 4, 2), (2, 512, 8, 64, 16, 1),
                                             (2, 256, 40, 48, 64, 1), (2, 256, 128, 32, 8, 1),
                                             (4, 512, 40, 96, 4, 4), (4, 1024, 64, 256, 64, 2), (2, 256, 16, 48, 8, 1),
@@ -353,7 +353,7 @@ def test_cb_every_variant_bit_exact(gpu, oracle, variant, F, T, A, B, ts, fs):
     np.testing.assert_array_equal(d_p.download(np.float64).reshape(F // fs, T // ts, B), p)
 
 
-@pytest.mark.parametrize("variant", [5, 8, 9, 10])
+@pytest.mark.parametrize("variant", [5, 8, 9, 10, 11])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_cb_quantiser_rounding_boundaries(gpu, oracle, variant, mode):
     """Levels that put many outputs exactly on (or next to) rounding
@@ -422,7 +422,7 @@ def test_cb_wave_private_staging_bit_exact(gpu, oracle, F, T, A, B, ts, mode):
     np.testing.assert_array_equal(d_o2.download(), ref)
 
 
-@pytest.mark.parametrize("variant", [0, 5, 9, 10])
+@pytest.mark.parametrize("variant", [0, 5, 9, 10, 11])
 def test_cb_set_levels_applies_to_later_blocks(gpu, oracle, variant):
     """New levels on a live plan (PipelineConfig's output level changing
     between blocks) apply to the next beamform call: the quantiser's

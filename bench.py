@@ -282,7 +282,11 @@ def corner_turn_leg(K, warmup):
     n = G * B * F * S * C
     d_in = capi.DeviceBuffer.from_array(capi.synth_gaussian_int8(7, 30.0, n), s)
     d_out = capi.DeviceBuffer(n)
-    legs = [("tuse_corner_turn", lambda: capi.tuse_transpose(d_in, d_out, C, S, F, B, G, stream=s),
+    # reference: a device-to-device copy of the same bytes (the HIP runtime's
+    # copy kernel), the practical ceiling of a read-once write-once kernel
+    legs = [("device_copy", lambda: capi.call("fbf_memcpy_d2d", d_out.ptr, d_in.ptr, n, s.handle),
+             "hipMemcpyAsync device to device of the same 256 MiB (ceiling reference)"),
+            ("tuse_corner_turn", lambda: capi.tuse_transpose(d_in, d_out, C, S, F, B, G, stream=s),
              "512 beams x 16 bands x 512 samples (4 groups of 128) x 64 channels, per-beam [group][sample][band][chan]"),
             ("feng_to_dada", lambda: capi.feng_to_dada(d_in, d_out, 256, stream=s),
              "1024 timestamps of [256 channels][256 samples] x 2 pols char2 -> [256 samples][256 channels]")]

@@ -858,6 +858,30 @@ __device__ __forceinline__ int8_t quantise_thr(long long P, const QuantConst& q,
     return (int8_t)(j - 128);
 }
 
+// The same with an FP32 estimate (about half the VALU time of the FP64
+// one).  For P >= 0, RN(RN(hi * 2^32) + RN(lo)) is within 3 * 2^-24 of P
+// relative, and the subtraction, product and RN(1 / scale) add at most
+// 2^-24 each relative to their results, so for |y| <= 129 the estimate is
+// within 2^-22 * (|offset / scale| + 258) of y: below one step when
+// |offset / scale| < 2^20 (thr_f32_applies), which keeps the estimate on
+// the right step or a neighbour; beyond |y| = 129 both saturate alike.
+__device__ __forceinline__ bool thr_f32_applies(float off, float sc)
+{
+    return fabsf(off / sc) < 1048576.0f;
+}
+
+__device__ __forceinline__ int8_t quantise_thr_f32(long long P, float off, float inv, const long long* thr, int mode)
+{
+    const float pf = (float)(int)(P >> 32) * 4294967296.0f + (float)(uint32_t)P;
+    const float y = (pf - off) * inv;
+    float e = mode ? truncf(y) : rintf(y);
+    e = fminf(fmaxf(e, -128.0f), 127.0f);
+    int j = (int)e + 128;
+    const long long lo = thr[j], hi = thr[j + 1];
+    j += (int)(P >= hi) - (int)(P < lo);
+    return (int8_t)(j - 128);
+}
+
 // One tile step of the transposed kernels: the RT tiles' MFMA chains and the
 // detection of their values into the pass accumulators pw[RT / 2][NSLOT]
 // (tile r adds to beam group r / 2; values of window q of the tile go to
@@ -906,10 +930,11 @@ struct TrRed {
 //
 // A work-group has 4 * NTS waves: NTS time slots (contiguous runs of passes
 // of the same channel group and beams) x 4 waves ((4 / FS) beam groups x FS
-// channels).  With NTS = 3 a work-group holds every wave of its compute unit
-// (three per SIMD), so the per-step barrier keeps them in lockstep: waves of
-// separate work-groups sharing a SIMD progress at different rates (the issue
-// arbiter prefers older waves), which leaves SIMDs with one wave at the end.
+// channels).  A work-group holds every wave of its compute unit (NTS = 3:
+// three per SIMD; NTS = 2 with RT = 8, whose waves need 256 VGPRs), so the
+// per-step barrier keeps them in lockstep: waves of separate work-groups
+// sharing a SIMD progress at different rates (the issue arbiter prefers
+// older waves), which leaves SIMDs with one wave at the end.
 template <int RT, int KSTEPS, int SPT, int NSLOT, int FS, int NTS>
 __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 {
@@ -1042,6 +1067,8 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     // barrier.
     const float q_off = p.offsets[fo], q_sc = p.scales[fo];
     const bool q_thr = thr_applies(q_off, q_sc);
+    const bool q_f32 = q_thr && thr_f32_applies(q_off, q_sc);
+    const float q_inv = 1.0f / q_sc;
     const QuantConst qz = quant_const(q_off, q_sc);
     if (tid < kThr) thr[tid] = thr_v;
     // Quantise and store one pass's totals (staged [fo][to][b] int8); with
@@ -1057,8 +1084,9 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
             const int b_out = RT == 2 ? beam_base + c : beam_base + 64 * (v / NSLOT) + 32 * h + c;
             const long long to = (long long)w0 + sl;
             if (!wave_active || b_out >= p.B || to >= p.nsamples_out) continue;
-            out_f[(size_t)to * p.B + b_out] = q_thr ? quantise_thr(t[v], qz, thr, p.quant)
-                                                    : quantise((double)t[v], q_off, q_sc, p.quant);
+            out_f[(size_t)to * p.B + b_out] =
+                q_f32 ? quantise_thr_f32(t[v], q_off, q_inv, thr, p.quant)
+                      : (q_thr ? quantise_thr(t[v], qz, thr, p.quant) : quantise((double)t[v], q_off, q_sc, p.quant));
         }
     };
     for (int pass = 0; pass < ta.npasses; ++pass) {

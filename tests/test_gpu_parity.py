@@ -439,10 +439,13 @@ def test_cb_set_levels_applies_to_later_blocks(gpu, oracle, variant):
     off1, sc1 = _levels(oracle, A, ts, 1, F)
     plan = capi.CbPlan(A, B, F, ts, 1, off1, sc1, nsamples_per_heap_out=T // ts)
     plan.set_variant(variant)
+    # the last set has |offset / scale| above 2^20 in half the channels,
+    # where the beamformer's quantiser takes its FP64 estimate
     levels = [(off1, sc1),
               (off1 * 1.5, sc1 * 0.25),
               (off1, np.where(np.arange(F) % 3 == 0, 0.0, sc1).astype(np.float32)),
-              (off1 * 0.5, sc1 * 3.0)]
+              (off1 * 0.5, sc1 * 3.0),
+              (off1, np.where(np.arange(F) % 2 == 0, sc1 * 1e-5, sc1).astype(np.float32))]
     for k, (off, sc) in enumerate(levels):
         if k:
             plan.set_levels(off, sc)

@@ -42,7 +42,7 @@ def _quantise_thr(P, off, sc, thr, mode):
 # (offset, scale): config-2 style levels, levels whose quotients are exact
 # integers and half-integers, large and tiny scales, a negative offset
 LEVELS = [(1.41e9, 2.6e7), (3.5, 1.0), (12.0, 8.0), (6.0e4, 2048.0), (7.0e12, 3.1e10),
-          (-2.5e5, 0.37), (0.0, 1e-3), (5.0e15, 1.0e13)]
+          (-2.5e5, 0.37), (0.0, 1e-3), (5.0e15, 1.0e13), (4194304.5, 1.0), (3.0e9, 7.0)]
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -62,6 +62,38 @@ def test_threshold_quantiser_matches_exact_division(oracle, off, sc, mode):
     np.testing.assert_array_equal(got, want)
     # the levels really cross steps: most thresholds are reachable
     assert reach.size > 100
+
+
+def _quantise_thr_f32(P, off, sc, thr, mode):
+    """The FP32 estimate of the coherent beamformer (used when
+    |offset / scale| < 2^20), corrected by the same thresholds."""
+    f32 = np.float32
+    pf = (P >> 32).astype(f32) * f32(4294967296.0) + (P & 0xFFFFFFFF).astype(f32)
+    y = (pf - f32(off)) * (f32(1.0) / f32(sc))
+    e = np.trunc(y) if mode else np.rint(y)
+    j = np.clip(e, -128, 127).astype(np.int64) + 128
+    j += (P >= thr[j + 1]).astype(np.int64) - (P < thr[j]).astype(np.int64)
+    return (j - 128).astype(np.int8)
+
+
+# |offset / scale| below 2^20, up to the bound
+LEVELS_F32 = [(1.41e9, 2.6e7), (3.5, 1.0), (12.0, 8.0), (-2.5e5, 0.37), (1.0e6 * 3.0, 3.0),
+              (-9.9e5 * 0.125, 0.125), (6.0e4, 2048.0)]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("off,sc", LEVELS_F32)
+def test_threshold_quantiser_f32_estimate_is_exact(oracle, off, sc, mode):
+    assert abs(np.float32(off) / np.float32(sc)) < 2 ** 20
+    thr = _thresholds(oracle, off, sc, mode)
+    reach = thr[1:256][thr[1:256] < INT64_MAX]
+    near = (reach[:, None] + np.arange(-3, 4)[None, :]).reshape(-1)
+    rng = np.random.default_rng(9)
+    spread = (rng.integers(0, 1 << 62, 20000) >> rng.integers(0, 62, 20000)).astype(np.int64)
+    P = np.concatenate([near[near >= 0], spread])
+    want = oracle.cb_tbtf(P.astype(np.float64).reshape(1, 1, -1), np.float32([off]), np.float32([sc]), 1,
+                          mode).reshape(-1)
+    np.testing.assert_array_equal(_quantise_thr_f32(P, off, sc, thr, mode), want)
 
 
 def test_threshold_estimate_alone_is_not_exact(oracle):

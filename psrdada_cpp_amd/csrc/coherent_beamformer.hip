@@ -761,12 +761,13 @@ __device__ __forceinline__ void dma16(const int8_t* src, int8_t* lds_run)
 // A wave-uniform pointer as scalar registers, so that a load from it plus a
 // 32-bit lane offset takes the scalar-base address form (no 64-bit vector
 // address arithmetic per load).
-__device__ __forceinline__ const int8_t* uniform_ptr(const int8_t* ptr)
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* ptr)
 {
     const unsigned long long v = reinterpret_cast<unsigned long long>(ptr);
     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-    return reinterpret_cast<const int8_t*>(((unsigned long long)hi << 32) | lo);
+    return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
 }
 
 // Work-group barrier without the memory fence of __syncthreads (which would
@@ -799,30 +800,39 @@ __device__ __forceinline__ long long swap32_sum(long long a, long long b)
     const unsigned blo = (uint32_t)(unsigned long long)b, bhi = (uint32_t)((unsigned long long)b >> 32);
     const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
     const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
-    const long long x = (long long)(((unsigned long long)hi[0] << 32) | lo[0]);
-    const long long y = (long long)(((unsigned long long)hi[1] << 32) | lo[1]);
-    return x + y;
+    // 64-bit sum as one add and one add-with-carry of the halves.
+    const unsigned slo = lo[0] + lo[1];
+    const unsigned shi = hi[0] + hi[1] + (slo < lo[0] ? 1u : 0u);
+    return (long long)(((unsigned long long)shi << 32) | slo);
 }
 
-// Offset and reciprocal scale of the threshold quantiser's estimate.
+// Constants of the threshold quantiser's estimates: FP64 (P - offset) * inv
+// with inv = RN(1 / scale), and FP32 P * inv_f + c_f with inv_f = RN(1 /
+// scale), c_f = RN(-offset * inv_f).
 struct QuantConst {
     double off_d, inv;
+    float inv_f, c_f;
 };
 
 __device__ __forceinline__ QuantConst quant_const(float off, float sc)
 {
-    return QuantConst{(double)off, 1.0 / (double)sc};
+    const float inv_f = 1.0f / sc;
+    return QuantConst{(double)off, 1.0 / (double)sc, inv_f, -off * inv_f};
 }
 
 // Quantiser thresholds.  For a positive, normal, finite scale and a finite
 // offset, quantise((double)P, offset, scale, mode) is a non-decreasing step
 // function of the integer power P: thr[j] is the least P >= 0 that quantises
 // to at least j - 128 (thr[0] = INT64_MIN, thr[256] = INT64_MAX, and
-// INT64_MAX where no P < 2^62 gets there).  The kernel's estimate
-// RN((P - offset) * RN(1 / scale)) is within 2^-43 of the quotient the exact
-// quantiser rounds (|y| <= 256; beyond, both saturate alike), so it lands on
-// the right step or a neighbour, and the two thresholds around it settle the
-// result without a division or a branch.
+// INT64_MAX where no P < 2^62 gets there).  The kernel needs only an
+// estimate e within one step of the result q: the two thresholds around e
+// then settle q without a division or a branch (P >= thr[e + 129] means
+// q = e + 1, P < thr[e + 128] means q = e - 1).  The estimate is
+// e = trunc(clamp(y, -128, 127)) for an approximation y of the quotient Y
+// that the exact quantiser rounds: whenever |y - Y| < 0.5, truncation moves
+// y by less than 1 and the mode's rounding moves Y by at most 0.5 (truncation
+// by less than 1, and then |y - Y| < 1 suffices), so |e - q| < 2 in either
+// mode (the clamp keeps this), i.e. e is q or a neighbour.
 constexpr int kThr = 257;
 
 __global__ __launch_bounds__(256) void cb_thresholds_k(const float* offsets, const float* scales, int mode,
@@ -858,39 +868,41 @@ __device__ __forceinline__ bool thr_applies(float off, float sc)
     return isfinite(off) && sc >= 1.17549435e-38f && sc <= 3.40282347e+38f;
 }
 
-__device__ __forceinline__ int8_t quantise_thr(long long P, const QuantConst& q, const long long* thr, int mode)
+// Threshold correction of an estimate e in [-128, 127]; thr_mid = thr + 128
+// (so thr_mid[e] and thr_mid[e + 1] are one ds_read2_b64).
+__device__ __forceinline__ int8_t thr_settle(long long P, int e, const long long* thr_mid)
 {
-    const double y = ((double)P - q.off_d) * q.inv;
-    double e = mode ? trunc(y) : rint(y);
-    e = fmin(fmax(e, -128.0), 127.0);
-    int j = (int)e + 128;
-    const long long lo = thr[j], hi = thr[j + 1];
-    j += (int)(P >= hi) - (int)(P < lo);
-    return (int8_t)(j - 128);
+    const long long lo = thr_mid[e], hi = thr_mid[e + 1];
+    return (int8_t)(e + (int)(P >= hi) - (int)(P < lo));
 }
 
-// The same with an FP32 estimate (about half the VALU time of the FP64
-// one).  For P >= 0, RN(RN(hi * 2^32) + RN(lo)) is within 3 * 2^-24 of P
-// relative, and the subtraction, product and RN(1 / scale) add at most
-// 2^-24 each relative to their results, so for |y| <= 129 the estimate is
-// within 2^-22 * (|offset / scale| + 258) of y: below one step when
-// |offset / scale| < 2^20 (thr_f32_applies), which keeps the estimate on
-// the right step or a neighbour; beyond |y| = 129 both saturate alike.
+// FP64 estimate RN(RN((double)P - offset) * RN(1 / scale)): it rounds the
+// same difference as the exact quantiser, so it is within 2^-43 of Y for
+// |Y| <= 256 at any offset and scale; beyond, both saturate alike.
+__device__ __forceinline__ int8_t quantise_thr(long long P, const QuantConst& q, const long long* thr_mid)
+{
+    const double y = ((double)P - q.off_d) * q.inv;
+    return thr_settle(P, (int)fmin(fmax(y, -128.0), 127.0), thr_mid);
+}
+
+// FP32 estimate (about a third of the VALU time of the FP64 one).  For
+// P >= 0, RN(RN(hi) * 2^32 + RN(lo)) is within 3 * 2^-24 of P relative;
+// RN(1 / scale) and c = RN(-offset * inv) are within 2^-24 and 2^-23
+// relative, and the fma rounds once, so for |Y| <= 130 the estimate is
+// within 2^-24 * (4 |P / scale| + 2 |offset / scale| + 131) <=
+// 2^-24 * (651 + 6 |offset / scale|) of Y: below 0.38 when
+// |offset / scale| < 2^20 (thr_f32_applies); beyond |Y| = 130 both
+// saturate alike.
 __device__ __forceinline__ bool thr_f32_applies(float off, float sc)
 {
     return fabsf(off / sc) < 1048576.0f;
 }
 
-__device__ __forceinline__ int8_t quantise_thr_f32(long long P, float off, float inv, const long long* thr, int mode)
+__device__ __forceinline__ int8_t quantise_thr_f32(long long P, const QuantConst& q, const long long* thr_mid)
 {
     const float pf = (float)(int)(P >> 32) * 4294967296.0f + (float)(uint32_t)P;
-    const float y = (pf - off) * inv;
-    float e = mode ? truncf(y) : rintf(y);
-    e = fminf(fmaxf(e, -128.0f), 127.0f);
-    int j = (int)e + 128;
-    const long long lo = thr[j], hi = thr[j + 1];
-    j += (int)(P >= hi) - (int)(P < lo);
-    return (int8_t)(j - 128);
+    const float y = __builtin_fmaf(pf, q.inv_f, q.c_f);
+    return thr_settle(P, (int)__builtin_amdgcn_fmed3f(y, -128.0f, 127.0f), thr_mid);
 }
 
 // One tile step of the transposed kernels: the RT tiles' MFMA chains and the
@@ -941,8 +953,8 @@ struct TrRed {
 //
 // A work-group has 4 * NTS waves: NTS time slots (contiguous runs of passes
 // of the same channel group and beams) x 4 waves ((4 / FS) beam groups x FS
-// channels).  A work-group holds every wave of its compute unit (NTS = 3:
-// three per SIMD; NTS = 2 with RT = 8, whose waves need 256 VGPRs), so the
+// channels).  A work-group holds every wave of its compute unit (NTS =
+// tr_nts waves per SIMD: 2, 3 or 4), so the
 // per-step barrier keeps them in lockstep: waves of separate work-groups
 // sharing a SIMD progress at different rates (the issue arbiter prefers
 // older waves), which leaves SIMDs with one wave at the end.
@@ -1083,25 +1095,31 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     const float q_off = p.offsets[fo], q_sc = p.scales[fo];
     const bool q_thr = thr_applies(q_off, q_sc);
     const bool q_f32 = q_thr && thr_f32_applies(q_off, q_sc);
-    const float q_inv = 1.0f / q_sc;
     const QuantConst qz = quant_const(q_off, q_sc);
     if (tid < kThr) thr[tid] = thr_v;
+    const long long* const thr_mid = thr + 128;
     // Quantise and store one pass's totals (staged [fo][to][b] int8); with
     // FS > 1 the waves of a beam group share the totals out.
     // Value v of a lane: slot 2v + h of beam c (RT = 2), else slot
-    // v % NSLOT of beam 64 (v / NSLOT) + 32 h + c.
+    // v % NSLOT of beam 64 (v / NSLOT) + 32 h + c.  The row of slot 2v
+    // (RT = 2) or v % NSLOT is wave-uniform, so its address is scalar and
+    // each lane adds a 32-bit offset (the next row for the upper half when
+    // RT = 2).
     int8_t* const out_f = static_cast<int8_t*>(p.out) + (size_t)fo * p.nsamples_out * p.B;
+    const int b_lane = RT == 2 ? beam_base + c : beam_base + 32 * h + c;
+    const int lane_off = (RT == 2 ? h * p.B : 0) + b_lane;
     auto store_pass = [&](const long long (&t)[VALS], int w0) {
 #pragma unroll
         for (int v = 0; v < VALS; ++v) {
             if (FS > 1 && v % FS != wc) continue;
-            const int sl = RT == 2 ? 2 * v + h : v % NSLOT;
-            const int b_out = RT == 2 ? beam_base + c : beam_base + 64 * (v / NSLOT) + 32 * h + c;
-            const long long to = (long long)w0 + sl;
-            if (!wave_active || b_out >= p.B || to >= p.nsamples_out) continue;
-            out_f[(size_t)to * p.B + b_out] =
-                q_f32 ? quantise_thr_f32(t[v], q_off, q_inv, thr, p.quant)
-                      : (q_thr ? quantise_thr(t[v], qz, thr, p.quant) : quantise((double)t[v], q_off, q_sc, p.quant));
+            const long long row = (long long)w0 + (RT == 2 ? 2 * v : v % NSLOT);
+            const int bsh = RT == 2 ? 0 : 64 * (v / NSLOT);
+            const long long to = row + (RT == 2 ? h : 0);
+            if (!wave_active || b_lane + bsh >= p.B || to >= p.nsamples_out) continue;
+            const int8_t q = q_f32 ? quantise_thr_f32(t[v], qz, thr_mid)
+                                   : (q_thr ? quantise_thr(t[v], qz, thr_mid)
+                                            : quantise((double)t[v], q_off, q_sc, p.quant));
+            uniform_ptr(out_f + (size_t)row * p.B + bsh)[lane_off] = q;
         }
     };
     for (int pass = 0; pass < ta.npasses; ++pass) {
@@ -1189,17 +1207,23 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 // tscrunch a power of two >= 4 (window 8, 16, 32 or a multiple of 32
 // columns), fscrunch 1, 2 or 4 with fscrunch * ceil(A / 16) <= 8, int8
 // output.
+// Windows per pass.  With 64 beams per wave and windows of 32 or more
+// columns, two windows per pass keep a wave within 128 VGPRs, so four waves
+// share a SIMD (tr_nts).
 inline int cb_tr_nslot(int rt, int W)
 {
     if (W <= 16) return rt == 2 ? 8 : 4;
-    return W == 32 ? 4 : 2;
+    return W == 32 && rt != 4 ? 4 : 2;
 }
 
 inline int cb_tr_nb(int ni) { return 60 / ni < 2 ? 2 : (60 / ni > 8 ? 8 : 60 / ni); }
 
-// Time slots per work-group of the transposed kernel: three waves per SIMD,
-// or two with 128 beams per wave (RT = 8: 256 VGPRs a wave).
-constexpr int tr_nts(int rt) { return rt == 8 ? 2 : 3; }
+// Time slots per work-group of the transposed kernel (waves per SIMD): four
+// with 64 beams per wave and two windows per pass (at most 128 VGPRs a
+// wave), two with 128 beams per wave (256 VGPRs), else three.  With four
+// waves a SIMD overlaps one wave's MFMAs with the others' detection better:
+// at 64 antennas the VALU work per step outweighs the MFMA work.
+constexpr int tr_nts(int rt, int nslot) { return rt == 8 ? 2 : (rt == 4 && nslot == 2 ? 4 : 3); }
 
 inline size_t cb_tr_shm(int fs, int ksteps, int vals, int nts)
 {
@@ -1219,7 +1243,7 @@ inline bool cb_tr_supported(int A, int ts, int fs, int rt)
     if (fs * ksteps > 8) return false;
     const int nslot = cb_tr_nslot(rt, 2 * ts);
     const int vals = rt == 2 ? nslot / 2 : (rt / 4) * nslot;
-    return cb_tr_shm(fs, ksteps, vals, tr_nts(rt)) <= 65536;
+    return cb_tr_shm(fs, ksteps, vals, tr_nts(rt, nslot)) <= 65536;
 }
 
 // Compute units of the current device (cached per device).
@@ -1244,7 +1268,7 @@ inline int device_cu_count()
 template <int RT, int KSTEPS, int SPT, int NSLOT, int FS>
 hipError_t launch_tr_k(CbTrArgs a, int passes, hipStream_t s)
 {
-    constexpr int NTS = tr_nts(RT);
+    constexpr int NTS = tr_nts(RT, NSLOT);
     const size_t shm = cb_tr_shm(FS, KSTEPS, TrRed<RT, NSLOT>::VALS, NTS);
     static int occ = 0;
     if (occ == 0) {
@@ -1301,7 +1325,7 @@ hipError_t launch_tr_w(const CbTrArgs& a, int W, int ksteps, int passes, hipStre
     constexpr int small = RT == 2 ? 8 : 4;
     if (W == 8) return launch_tr_ks<RT, 4, small, FS>(a, ksteps, passes, s);
     if (W == 16) return launch_tr_ks<RT, 2, small, FS>(a, ksteps, passes, s);
-    if (W == 32) return launch_tr_ks<RT, 1, 4, FS>(a, ksteps, passes, s);
+    if (W == 32) return launch_tr_ks<RT, 1, RT == 4 ? 2 : 4, FS>(a, ksteps, passes, s);
     return launch_tr_ks<RT, 1, 2, FS>(a, ksteps, passes, s);
 }
 
@@ -1505,7 +1529,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void cb_ws_k(CbWsArgs wa)
     int fo = 0, beam_base = 0;
     float q_off = 0.f, q_sc = 1.f;
     bool q_thr = false;
-    QuantConst qz{0.0, 1.0};
+    QuantConst qz{0.0, 1.0, 1.0f, 0.0f};
     const long long* thr = thr_all;
 
     int q = 0;   // sequence position of the compute cursor
@@ -1581,7 +1605,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void cb_ws_k(CbWsArgs wa)
                 const long long to = (long long)win0 + v;
                 if (!wave_active || b_out >= p.B || to >= p.nsamples_out) continue;
                 int8_t* row = static_cast<int8_t*>(p.out) + ((size_t)fo * p.nsamples_out + (size_t)to) * p.B;
-                row[b_out] = q_thr ? quantise_thr(tot[v], qz, thr, p.quant)
+                row[b_out] = q_thr ? quantise_thr(tot[v], qz, thr + 128)
                                    : quantise((double)tot[v], q_off, q_sc, p.quant);
             }
         }
@@ -1634,7 +1658,9 @@ hipError_t launch_ws_ks(const CbWsArgs& a, int ksteps, hipStream_t s)
     }
 }
 
-// Windows per pass as cb_tr_nslot with RT = 4.
+// Windows per pass of cb_ws_k.
+inline int cb_ws_nslot(int W) { return W <= 32 ? 4 : 2; }
+
 inline hipError_t launch_ws(const CbWsArgs& a, int W, int ksteps, hipStream_t s)
 {
     if (W == 8) return launch_ws_ks<4, 4>(a, ksteps, s);
@@ -2161,7 +2187,7 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
         const int W = 2 * d.tscrunch;
         const int spt = W >= 32 ? 1 : 32 / W;
         wa.tps = W >= 32 ? W / 32 : 1;
-        wa.ntp = (cb_tr_nslot(4, W) / spt) * wa.tps;
+        wa.ntp = (cb_ws_nslot(W) / spt) * wa.tps;
         wa.ntiles = (int)(2 * T / 32);
         wa.nbg = (d.nbeams + 63) / 64;
         const int ksteps = (2 * d.nantennas + 31) / 32;

@@ -1,10 +1,11 @@
 """The coherent beamformer's quantiser (cb_tr_k) replaces the division of the
 exact quantiser by a per-channel table: thr[j] is the least integer power
-that quantises to at least j - 128, and a quotient estimate
-RN((P - offset) * RN(1 / scale)) is corrected by the two thresholds around
-it.  This checks that arithmetic on the CPU (numpy float64 is IEEE double,
-as on the device) against the oracle's quantiser, with thresholds found by
-binary search on the oracle itself; the GPU tests check the kernel.
+that quantises to at least j - 128.  An estimate e = trunc(clamp(y)) of the
+quotient, within one step of the result in either rounding mode, is
+corrected by the two thresholds around it.  This checks that arithmetic on
+the CPU (numpy float64 and float32 are IEEE, as on the device) against the
+oracle's quantiser, with thresholds found by binary search on the oracle
+itself; the GPU tests check the kernel.
 """
 import numpy as np
 import pytest
@@ -31,12 +32,18 @@ def _thresholds(oracle, off, sc, mode):
     return t
 
 
-def _quantise_thr(P, off, sc, thr, mode):
-    y = (P.astype(np.float64) - np.float64(np.float32(off))) * (1.0 / np.float64(np.float32(sc)))
-    e = np.trunc(y) if mode else np.rint(y)
-    j = np.clip(e, -128, 127).astype(np.int64) + 128
+def _settle(P, e, thr):
+    """Threshold correction of the estimate e in [-128, 127]."""
+    j = e.astype(np.int64) + 128
     j += (P >= thr[j + 1]).astype(np.int64) - (P < thr[j]).astype(np.int64)
     return (j - 128).astype(np.int8)
+
+
+def _quantise_thr(P, off, sc, thr, mode):
+    """FP64 estimate RN(RN(P - offset) * RN(1 / scale)), truncated whatever
+    the mode (the correction absorbs the rounding)."""
+    y = (P.astype(np.float64) - np.float64(np.float32(off))) * (1.0 / np.float64(np.float32(sc)))
+    return _settle(P, np.trunc(np.clip(y, -128, 127)), thr)
 
 
 # (offset, scale): config-2 style levels, levels whose quotients are exact
@@ -66,14 +73,16 @@ def test_threshold_quantiser_matches_exact_division(oracle, off, sc, mode):
 
 def _quantise_thr_f32(P, off, sc, thr, mode):
     """The FP32 estimate of the coherent beamformer (used when
-    |offset / scale| < 2^20), corrected by the same thresholds."""
+    |offset / scale| < 2^20): fma(pf, RN(1 / scale), RN(-offset * inv)),
+    truncated, corrected by the same thresholds.  The fma is modelled by the
+    exact product in float64 plus the constant, rounded to float64 and then
+    to float32 (one rounding more than the device's fma)."""
     f32 = np.float32
     pf = (P >> 32).astype(f32) * f32(4294967296.0) + (P & 0xFFFFFFFF).astype(f32)
-    y = (pf - f32(off)) * (f32(1.0) / f32(sc))
-    e = np.trunc(y) if mode else np.rint(y)
-    j = np.clip(e, -128, 127).astype(np.int64) + 128
-    j += (P >= thr[j + 1]).astype(np.int64) - (P < thr[j]).astype(np.int64)
-    return (j - 128).astype(np.int8)
+    inv = f32(1.0) / f32(sc)
+    c = -f32(off) * inv
+    y = (pf.astype(np.float64) * np.float64(inv) + np.float64(c)).astype(f32)
+    return _settle(P, np.trunc(np.clip(y, -128, 127)), thr)
 
 
 # |offset / scale| below 2^20, up to the bound
@@ -94,6 +103,18 @@ def test_threshold_quantiser_f32_estimate_is_exact(oracle, off, sc, mode):
     want = oracle.cb_tbtf(P.astype(np.float64).reshape(1, 1, -1), np.float32([off]), np.float32([sc]), 1,
                           mode).reshape(-1)
     np.testing.assert_array_equal(_quantise_thr_f32(P, off, sc, thr, mode), want)
+    # the estimate's error bound, 2^-24 (651 + 6 |offset / scale|) for
+    # |Y| <= 130, stays below 0.5 (so the truncated estimate is within a step)
+    f32 = np.float32
+    o = abs(float(f32(off)) / float(f32(sc)))
+    pf = (P >> 32).astype(f32) * f32(4294967296.0) + (P & 0xFFFFFFFF).astype(f32)
+    inv = f32(1.0) / f32(sc)
+    y = (pf.astype(np.float64) * np.float64(inv) + np.float64(-f32(off) * inv)).astype(f32).astype(np.float64)
+    Y = (P.astype(np.float64) - float(f32(off))) / float(f32(sc))
+    near = np.abs(Y) <= 130     # empty for levels where every P >= 0 saturates
+    bound = 2.0 ** -24 * (651 + 6 * o)
+    assert bound < 0.5
+    assert np.all(np.abs(y - Y)[near] <= bound)
 
 
 def test_threshold_estimate_alone_is_not_exact(oracle):

@@ -770,6 +770,13 @@ __device__ __forceinline__ T* uniform_ptr(T* ptr)
     return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
 }
 
+// A pointer known to address global memory, so that stores through it are
+// global (not flat: a flat store also counts against the LDS wait counter).
+__device__ __forceinline__ __attribute__((address_space(1))) int8_t* global_ptr(int8_t* ptr)
+{
+    return (__attribute__((address_space(1))) int8_t*)ptr;
+}
+
 // Work-group barrier without the memory fence of __syncthreads (which would
 // wait for every outstanding global load, including the LDS prefetches).
 // Callers wait for their own LDS and global->LDS traffic first; the empty
@@ -1108,18 +1115,22 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     int8_t* const out_f = static_cast<int8_t*>(p.out) + (size_t)fo * p.nsamples_out * p.B;
     const int b_lane = RT == 2 ? beam_base + c : beam_base + 32 * h + c;
     const int lane_off = (RT == 2 ? h * p.B : 0) + b_lane;
+    // Output windows number below 2^31 (cb_tr_applies), so window indices
+    // compare as int: a scalar compare for the wave-uniform rows.
+    const int nout = (int)p.nsamples_out;
     auto store_pass = [&](const long long (&t)[VALS], int w0) {
 #pragma unroll
         for (int v = 0; v < VALS; ++v) {
             if (FS > 1 && v % FS != wc) continue;
-            const long long row = (long long)w0 + (RT == 2 ? 2 * v : v % NSLOT);
+            const int row = w0 + (RT == 2 ? 2 * v : v % NSLOT);
             const int bsh = RT == 2 ? 0 : 64 * (v / NSLOT);
-            const long long to = row + (RT == 2 ? h : 0);
-            if (!wave_active || b_lane + bsh >= p.B || to >= p.nsamples_out) continue;
+            // Uniform bounds first (a scalar branch), then the lane's.
+            if (!wave_active || row >= nout) continue;
             const int8_t q = q_f32 ? quantise_thr_f32(t[v], qz, thr_mid)
                                    : (q_thr ? quantise_thr(t[v], qz, thr_mid)
                                             : quantise((double)t[v], q_off, q_sc, p.quant));
-            uniform_ptr(out_f + (size_t)row * p.B + bsh)[lane_off] = q;
+            if (b_lane + bsh < p.B && (RT != 2 || row + h < nout))
+                global_ptr(uniform_ptr(out_f + (size_t)row * p.B + bsh))[(unsigned)lane_off] = q;
         }
     };
     for (int pass = 0; pass < ta.npasses; ++pass) {
@@ -2049,7 +2060,7 @@ constexpr int kCbVariantWs = 10;
 
 static bool cb_tr_applies(const fbf_cb_desc& d, int rt, int mode, long long T)
 {
-    return mode == MODE_INT8 && T % 16 == 0 && 2 * T / 32 < (1ll << 31) &&
+    return mode == MODE_INT8 && T % 16 == 0 && 2 * T / 32 < (1ll << 31) && T / d.tscrunch < (1ll << 31) &&
            cb_tr_supported(d.nantennas, d.tscrunch, d.fscrunch, rt);
 }
 

@@ -886,10 +886,15 @@ __device__ __forceinline__ int8_t thr_settle(long long P, int e, const long long
 // FP64 estimate RN(RN((double)P - offset) * RN(1 / scale)): it rounds the
 // same difference as the exact quantiser, so it is within 2^-43 of Y for
 // |Y| <= 256 at any offset and scale; beyond, both saturate alike.
-__device__ __forceinline__ int8_t quantise_thr(long long P, const QuantConst& q, const long long* thr_mid)
+__device__ __forceinline__ int thr_estimate(long long P, const QuantConst& q)
 {
     const double y = ((double)P - q.off_d) * q.inv;
-    return thr_settle(P, (int)fmin(fmax(y, -128.0), 127.0), thr_mid);
+    return (int)fmin(fmax(y, -128.0), 127.0);
+}
+
+__device__ __forceinline__ int8_t quantise_thr(long long P, const QuantConst& q, const long long* thr_mid)
+{
+    return thr_settle(P, thr_estimate(P, q), thr_mid);
 }
 
 // FP32 estimate (about a third of the VALU time of the FP64 one).  For
@@ -905,11 +910,11 @@ __device__ __forceinline__ bool thr_f32_applies(float off, float sc)
     return fabsf(off / sc) < 1048576.0f;
 }
 
-__device__ __forceinline__ int8_t quantise_thr_f32(long long P, const QuantConst& q, const long long* thr_mid)
+__device__ __forceinline__ int thr_estimate_f32(long long P, const QuantConst& q)
 {
     const float pf = (float)(int)(P >> 32) * 4294967296.0f + (float)(uint32_t)P;
     const float y = __builtin_fmaf(pf, q.inv_f, q.c_f);
-    return thr_settle(P, (int)__builtin_amdgcn_fmed3f(y, -128.0f, 127.0f), thr_mid);
+    return (int)__builtin_amdgcn_fmed3f(y, -128.0f, 127.0f);
 }
 
 // One tile step of the transposed kernels: the RT tiles' MFMA chains and the
@@ -1118,7 +1123,25 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     // Output windows number below 2^31 (cb_tr_applies), so window indices
     // compare as int: a scalar compare for the wave-uniform rows.
     const int nout = (int)p.nsamples_out;
+    // All of a pass's values are estimated first, then their thresholds
+    // read (the LDS reads in flight together, one wait), then stored.
     auto store_pass = [&](const long long (&t)[VALS], int w0) {
+        int8_t q[VALS];
+        if (q_thr) {
+            int e[VALS];
+            if (q_f32) {
+#pragma unroll
+                for (int v = 0; v < VALS; ++v) e[v] = thr_estimate_f32(t[v], qz);
+            } else {
+#pragma unroll
+                for (int v = 0; v < VALS; ++v) e[v] = thr_estimate(t[v], qz);
+            }
+#pragma unroll
+            for (int v = 0; v < VALS; ++v) q[v] = thr_settle(t[v], e[v], thr_mid);
+        } else {
+#pragma unroll
+            for (int v = 0; v < VALS; ++v) q[v] = quantise((double)t[v], q_off, q_sc, p.quant);
+        }
 #pragma unroll
         for (int v = 0; v < VALS; ++v) {
             if (FS > 1 && v % FS != wc) continue;
@@ -1126,11 +1149,8 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
             const int bsh = RT == 2 ? 0 : 64 * (v / NSLOT);
             // Uniform bounds first (a scalar branch), then the lane's.
             if (!wave_active || row >= nout) continue;
-            const int8_t q = q_f32 ? quantise_thr_f32(t[v], qz, thr_mid)
-                                   : (q_thr ? quantise_thr(t[v], qz, thr_mid)
-                                            : quantise((double)t[v], q_off, q_sc, p.quant));
             if (b_lane + bsh < p.B && (RT != 2 || row + h < nout))
-                global_ptr(uniform_ptr(out_f + (size_t)row * p.B + bsh))[(unsigned)lane_off] = q;
+                global_ptr(uniform_ptr(out_f + (size_t)row * p.B + bsh))[(unsigned)lane_off] = q[v];
         }
     };
     for (int pass = 0; pass < ta.npasses; ++pass) {

@@ -476,6 +476,22 @@ def cpu_baseline(cfg, sub, taftp, nch):
     t3 = time.perf_counter()
     del q
     block_cpu_s = (t1 - t0) + (t2 - t1) + (t3 - t2) * (F / nch)
+    # One core: IB over the whole block and CB over 4 channels, 1 thread.
+    n1 = min(4, nch)
+    u0 = time.perf_counter()
+    oracle.ib_power(taftp, F, cfg.ib_tscrunch, cfg.ib_fscrunch, nhg, cfg.total_nantennas, nthreads=1)
+    u1 = time.perf_counter()
+    p1 = oracle.cb_power(ftpa[: n1 * T * 2 * A * 2], w[: n1 * B * A * 2], n1, T, A, B, sub.tscrunch, sub.fscrunch,
+                         nthreads=1)
+    oracle.cb_tbtf(p1, np.ones(n1, np.float32), np.ones(n1, np.float32), T // sub.tscrunch)
+    u2 = time.perf_counter()
+    block_1core_s = (t1 - t0) + (u1 - u0) + (u2 - u1) * (F / n1)
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": round(cfg.block_seconds / block_cpu_s, 4),
         "unit": "x real-time",
@@ -486,6 +502,11 @@ def cpu_baseline(cfg, sub, taftp, nch):
                   f"measured {t3 - t0:.1f} s, extrapolated block time {block_cpu_s:.2f} s",
         "int8_tops": round(cfg.cb_ops() / block_cpu_s / 1e12, 4),
         "cb_seconds_per_block": round((t3 - t2) * (F / nch), 3),
+        "cpu_model": cpu_model,
+        "host_cpus": os.cpu_count(),
+        "one_core": {"value": round(cfg.block_seconds / block_1core_s, 5), "unit": "x real-time",
+                     "sample": f"split-transpose and IB over the whole block, CB over {n1} channels scaled "
+                               f"x{F // n1}, 1 thread; extrapolated block time {block_1core_s:.1f} s"},
     }
 
 

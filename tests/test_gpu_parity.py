@@ -382,6 +382,40 @@ def test_cb_quantiser_rounding_boundaries(gpu, oracle, variant, mode):
     assert np.any(p[0] % 2 == 1)
 
 
+def test_cb_f32_estimate_at_its_bound(gpu, oracle):
+    """Levels with |offset / scale| just below 2^20, the bound under which the
+    transposed kernels estimate the quantiser's quotient in FP32 (two channels
+    just above it take the FP64 estimate).  Each channel's offset is one of
+    its own powers (that output's quotient is 0) and its scale is offset / r,
+    so only powers within 128 * scale of the offset (about 1e-4 relative)
+    are not saturated: a few per channel, the rest exercise saturation at
+    both ends.  Both rounding modes, the kernel with 64 and with 128 beams
+    per wave; every output must match the oracle."""
+    capi = gpu
+    F, T, A, B, ts = 16, 4096, 32, 64, 16
+    rng = np.random.default_rng(2020)
+    ftpa = gaussian_int8(2021, F * T * 2 * A * 2, 24.0)
+    w = rng.integers(-127, 128, F * B * A * 2).astype(np.int8)
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, 1)
+    r = np.concatenate([np.linspace(0.999e6, 1.0485e6, F - 2), [1.0487e6, 1.2e6]])
+    off = np.array([np.median(p[f]) for f in range(F)], np.float32)
+    off = np.array([p[f].reshape(-1)[np.argmin(np.abs(p[f].reshape(-1) - off[f]))] for f in range(F)], np.float32)
+    sc = (off.astype(np.float64) / r).astype(np.float32)
+    below = np.abs(off / sc) < np.float32(2 ** 20)
+    assert below[:F - 2].all() and not below[F - 2:].any()
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    d_o = capi.DeviceBuffer(F * (T // ts) * B)
+    for mode in (0, 1):
+        ref = oracle.cb_tbtf(p, off, sc, T // ts, mode)
+        inside = np.count_nonzero((ref > -128) & (ref < 127))
+        assert inside >= 2 * F   # the unsaturated band is reached
+        for variant in (0, 9, 11):
+            plan = capi.CbPlan(A, B, F, ts, 1, off, sc, nsamples_per_heap_out=T // ts, quant_mode=mode)
+            plan.set_variant(variant)
+            plan.beamform(d_v, d_w, d_o)
+            np.testing.assert_array_equal(d_o.download(), ref, err_msg=f"mode {mode} variant {variant}")
+
+
 @pytest.mark.parametrize("F,T,A,B,ts,mode", [
     (4, 2048, 64, 512, 16, 0),     # 8 units: time ranges, one unit per work-group
     (37, 256, 16, 576, 16, 1),     # 333 units: two per work-group, the last work-group holds one

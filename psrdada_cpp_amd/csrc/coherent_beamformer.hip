@@ -1238,11 +1238,13 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 // tscrunch a power of two >= 4 (window 8, 16, 32 or a multiple of 32
 // columns), fscrunch 1, 2 or 4 with fscrunch * ceil(A / 16) <= 8, int8
 // output.
-// Windows per pass.  With 64 beams per wave and windows of 32 or more
+// Windows per pass.  With 64 beams per wave and windows of 16 or more
 // columns, two windows per pass keep a wave within 128 VGPRs, so four waves
-// share a SIMD (tr_nts).
+// share a SIMD (tr_nts).  Windows of 8 columns fill a tile four times, so a
+// pass holds at least 4.
 inline int cb_tr_nslot(int rt, int W)
 {
+    if (W == 16 && rt == 4) return 2;
     if (W <= 16) return rt == 2 ? 8 : 4;
     return W == 32 && rt != 4 ? 4 : 2;
 }
@@ -1355,7 +1357,7 @@ hipError_t launch_tr_w(const CbTrArgs& a, int W, int ksteps, int passes, hipStre
     // Windows per pass as cb_tr_nslot
     constexpr int small = RT == 2 ? 8 : 4;
     if (W == 8) return launch_tr_ks<RT, 4, small, FS>(a, ksteps, passes, s);
-    if (W == 16) return launch_tr_ks<RT, 2, small, FS>(a, ksteps, passes, s);
+    if (W == 16) return launch_tr_ks<RT, 2, RT == 4 ? 2 : small, FS>(a, ksteps, passes, s);
     if (W == 32) return launch_tr_ks<RT, 1, RT == 4 ? 2 : 4, FS>(a, ksteps, passes, s);
     return launch_tr_ks<RT, 1, 2, FS>(a, ksteps, passes, s);
 }

@@ -189,7 +189,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8 (int32 accumulate, fp64 detect)",
+        "dtype": "int8 (int32 accumulate, exact int64 detect)",
         "data": "synthetic: TAFTP voltages round(N(0,16)) clipped to +-127, seeded splitmix64; "
                 "weights from tau ~ U(-1us, 1us) via the FP64-phase generator",
         "config": {

@@ -329,7 +329,8 @@ def test_weights_fp64_phase_config4(gpu, oracle, epoch):
 @pytest.mark.parametrize("F,T,A,B,ts,fs", [(4, 512, 64, 512, 16, 1), (4, 256, 44, 96, 4, 2), (2, 512, 8, 64, 16, 1),
                                             (2, 256, 40, 48, 64, 1), (2, 256, 128, 32, 8, 1),
                                             (4, 512, 40, 96, 4, 4), (4, 1024, 64, 256, 64, 2), (2, 256, 16, 48, 8, 1),
-                                            (8, 256, 24, 72, 32, 4), (4, 512, 64, 136, 16, 2)])
+                                            (8, 256, 24, 72, 32, 4), (4, 512, 64, 136, 16, 2),
+                                            (4, 1536, 64, 256, 8, 1), (3, 1040, 48, 200, 16, 1)])
 def test_cb_every_variant_bit_exact(gpu, oracle, variant, F, T, A, B, ts, fs):
     """Every kernel variant (MFMA shape x tile) gives the oracle's sums and power
     (the transposed variants 8 and 9 serve int8 and power output and fall back

@@ -129,3 +129,26 @@ def test_threshold_estimate_alone_is_not_exact(oracle):
                           mode).reshape(-1)
     assert np.any(est != want)
     np.testing.assert_array_equal(_quantise_thr(P, off, sc, thr, mode), want)
+
+
+def test_fp64_estimate_needs_the_difference_first(oracle):
+    """The FP64 estimate rounds (P - offset) before scaling, as the exact
+    quantiser does, so its error is relative to the small quotient.  The
+    single-fma form P * inv - offset * inv (right for the FP32 path, whose
+    |offset / scale| is bounded) has an error of about 2^-52 |offset /
+    scale|: at offset 3 * 2^58, scale 60 that is several steps, and its
+    corrected estimate misses the exact result for many powers."""
+    off, sc, mode = 3.0 * 2.0 ** 58, 60.0, 0
+    thr = _thresholds(oracle, off, sc, mode)
+    reach = thr[1:256][thr[1:256] < INT64_MAX]
+    assert reach.size > 100
+    P = (reach[:, None] + np.arange(-40, 41, 8)[None, :]).reshape(-1)
+    P = P[P >= 0]
+    want = oracle.cb_tbtf(P.astype(np.float64).reshape(1, 1, -1), np.float32([off]), np.float32([sc]), 1,
+                          mode).reshape(-1)
+    np.testing.assert_array_equal(_quantise_thr(P, off, sc, thr, mode), want)
+    inv = 1.0 / np.float64(np.float32(sc))
+    c = -np.float64(np.float32(off)) * inv
+    y_fma = P.astype(np.float64) * inv + c      # one rounding more than an fma: errors of the same size
+    wrong = _settle(P, np.trunc(np.clip(y_fma, -128, 127)), thr)
+    assert np.count_nonzero(wrong != want) > 100

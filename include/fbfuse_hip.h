@@ -133,6 +133,9 @@ typedef struct fbf_cb_plan fbf_cb_plan;
 /* offsets/scales: host arrays of F / fscrunch floats (per output channel). */
 int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets, const float* scales,
                        fbf_cb_plan** plan);
+/* New per-channel levels for later calls.  Waits for all work on the device
+ * first (a beamform in flight finishes with the old levels), so it must not
+ * be called while a stream is being captured. */
 int fbf_cb_plan_set_levels(fbf_cb_plan* plan, const float* offsets, const float* scales);
 int fbf_cb_plan_destroy(fbf_cb_plan* plan);
 /* Kernel tile variant (0 = autotuned default); see fbf_cb_variant_count. */
@@ -198,6 +201,7 @@ typedef struct fbf_ib_plan fbf_ib_plan;
 
 int fbf_ib_plan_create(const fbf_ib_desc* desc, const float* offsets, const float* scales,
                        fbf_ib_plan** plan);
+/* As fbf_cb_plan_set_levels: waits for all work on the device first. */
 int fbf_ib_plan_set_levels(fbf_ib_plan* plan, const float* offsets, const float* scales);
 int fbf_ib_plan_destroy(fbf_ib_plan* plan);
 int fbf_ib_output_bytes(const fbf_ib_plan* plan, size_t taftp_bytes, size_t* tf_bytes);

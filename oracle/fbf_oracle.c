@@ -55,13 +55,10 @@ void oracle_split_transpose(const int8_t* taftp, int8_t* ftpa,
 /* Row Re_b = [wr_a, -wi_a], row Im_b = [wi_a, wr_a] over k = (a, re/im)*/
 /* so that Re = sum wr*vr - wi*vi and Im = sum wi*vr + wr*vi, which is  */
 /* r = xx - yy and i = xy + yx of CoherentBeamformer.cu:142-156 and     */
-/* CoherentBeamformerTester.cu:88-93.  The negation saturates          */
-/* (-(-128) = 127); weights round(127 e^{i theta}) never hold -128.     */
+/* CoherentBeamformerTester.cu:88-93.  -wi is formed in int16, so it is */
+/* exact for every int8 weight (128 for wi = -128), as the reference's  */
+/* int32 dp4a arithmetic is.                                            */
 /* ------------------------------------------------------------------ */
-static inline int16_t sat_neg(int8_t x)
-{
-    return x == -128 ? (int16_t)127 : (int16_t)(-x);
-}
 
 static void embed_channel(const int8_t* fba_f, int nbeams, int nantennas,
                           int16_t* wre, int16_t* wim)
@@ -71,7 +68,7 @@ static void embed_channel(const int8_t* fba_f, int nbeams, int nantennas,
             int8_t wr = fba_f[2 * ((long)b * nantennas + a)];
             int8_t wi = fba_f[2 * ((long)b * nantennas + a) + 1];
             wre[(long)b * 2 * nantennas + 2 * a] = wr;
-            wre[(long)b * 2 * nantennas + 2 * a + 1] = sat_neg(wi);
+            wre[(long)b * 2 * nantennas + 2 * a + 1] = (int16_t)(-(int16_t)wi);
             wim[(long)b * 2 * nantennas + 2 * a] = wi;
             wim[(long)b * 2 * nantennas + 2 * a + 1] = wr;
         }
@@ -313,9 +310,10 @@ void oracle_ib_reference_f32(const int8_t* taftp, int8_t* tf, int nchans,
 /* ------------------------------------------------------------------ */
 /* Weights: WeightsManager.cu:58-73, WeightsManagerTester.cu:38-73      */
 /* ------------------------------------------------------------------ */
+/* Nearest integer, ties to even: __float2int_rn (WeightsManager.cu:70-71). */
 static int8_t round_to_int8(double v)
 {
-    double r = round(v);
+    double r = rint(v);
     if (r > 127.0)
         r = 127.0;
     if (r < -128.0)
@@ -363,8 +361,8 @@ void oracle_weights_reference_f32(const float* delays, const float* freqs,
                 float s = sinf(arg), co = cosf(arg);
                 volatile float xr = co * 127.0f, xi = s * 127.0f;
                 long oi = (long)nantennas * ((long)nbeams * c + b) + a;
-                weights[2 * oi] = (int8_t)roundf(xr);
-                weights[2 * oi + 1] = (int8_t)roundf(xi);
+                weights[2 * oi] = (int8_t)rintf(xr);
+                weights[2 * oi + 1] = (int8_t)rintf(xi);
             }
         }
 }

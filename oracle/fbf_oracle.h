@@ -20,8 +20,8 @@
  *     or FBF_QUANT_TRUNC: truncate toward zero (the reference's
  *     (int8_t) cast, CoherentBeamformer.cu:169), both saturated to
  *     [-128, 127].
- *   * the negated weight component in the real embedding saturates:
- *     -(-128) -> 127 (see oracle_cb_sums).
+ *   * the int32 beam sums are exact for every int8 weight, -128 included,
+ *     as the reference's dp4a arithmetic is (CoherentBeamformer.cu:150-155).
  */
 #ifndef FBF_ORACLE_H
 #define FBF_ORACLE_H

@@ -21,7 +21,10 @@
 // voltage column is one contiguous 16-byte load straight into the MFMA B
 // operand -- no de-interleave and no LDS staging.  The weights are embedded in
 // registers from the raw FBA char2 bytes (byte swap for Im rows, saturating
-// negation of the odd bytes for Re rows), once per channel per wave.
+// negation of the odd bytes for Re rows), once per channel per wave; waves
+// whose weights hold a wi = -128, where the negation saturates, take an exact
+// path instead (cb_exact_outputs), so the sums are exact for every int8
+// weight.
 //
 // v_mfma_i32_16x16x64_i8: A (16 rows x 64 k) = weights, B (64 k x 16 cols) =
 // voltages, D lane l holds rows 4*(l/16)+i, column l%16.  Rows are ordered
@@ -138,6 +141,115 @@ __device__ __forceinline__ v4i embed_slice(v4i raw, bool im_row)
     return e;
 }
 
+// Weight components wi = -128.  The Re row carries sat(-wi), which is 127
+// for wi = -128, one short of -wi = 128, while the reference's int32 sums
+// (r = xx - yy, CoherentBeamformer.cu:150-155) are exact for every int8
+// weight.  Weights round(127 e^{i theta}) never hold -128, so the MFMA paths
+// stay as they are and every kernel first checks the weights it will use:
+// a wave (or a work-group, where its waves share staging) that finds a
+// wi = -128 computes its outputs with cb_exact_outputs instead.
+//
+// Flag dword from a dword x: 0x01 at the odd byte of every 16-bit pair of x
+// whose even byte is 0x80, else 0.  e's even byte is zero exactly for 0x80,
+// and adding 0xFF to it carries into bit 8 exactly when it is not zero (at
+// most 0x1FE, so the pairs do not interact).
+__device__ __forceinline__ int m128_corr(uint32_t x)
+{
+    const uint32_t e = (x & 0x00FF00FFu) ^ 0x00800080u;
+    return (int)(((e + 0x00FF00FFu) & 0x01000100u) ^ 0x01000100u);
+}
+
+// Whether a raw FBA slice [wr, wi, ...] holds a wi = -128 (an odd byte 0x80).
+__device__ __forceinline__ bool raw_has_m128(v4i raw)
+{
+    int any = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) any |= m128_corr((uint32_t)raw[q] >> 8);
+    return any != 0;
+}
+
+// The same over a wave: uniform, so the exact path is a scalar branch.
+__device__ __forceinline__ bool wave_any(bool x)
+{
+    return __builtin_amdgcn_ballot_w64(x) != 0;
+}
+
+// Exact outputs of output channel fo for beams [b0, b0 + nb) and output
+// windows [w0, w1): the int32 sums of every column (MODE_SUMS,
+// [F][T][P][B][2]), the power (MODE_POWER, double [Fo][To][B]) or the staged
+// int8 output (MODE_INT8, [Fo][To][B]).  Thread t of nt takes (beam, window)
+// pairs t, t + nt, ... and forms plain int32 dot products:
+// r = sum wr vr - wi vi and i = sum wr vi + wi vr are exact for every int8
+// weight (CoherentBeamformer.cu:150-155); the power is exact in int64 and the
+// quantiser is the exact one.  The path of waves (or work-groups) whose
+// weights hold a wi = -128; consecutive threads take consecutive beams, so a
+// wave's voltage reads are one broadcast column.
+template <int MODE>
+__device__ __forceinline__ void cb_exact_outputs(const CbArgs& p, int fo, int b0, int nb, long long w0,
+                                                 long long w1, int t, int nt)
+{
+    if (w1 > p.nsamples_out) w1 = p.nsamples_out;
+    if (w1 <= w0) return;
+    const int K = 2 * p.A;
+    const size_t TP = (size_t)2 * p.nsamples;
+    const int W = 2 * p.ts;
+    const float off = p.offsets[fo], sc = p.scales[fo];
+    const long long n = (long long)nb * (w1 - w0);
+    for (long long idx = t; idx < n; idx += nt) {
+        const int b = b0 + (int)(idx % nb);
+        const long long w = w0 + idx / nb;
+        if (b >= p.B) continue;
+        long long P = 0;
+        for (int fc = 0; fc < p.fs; ++fc) {
+            const int f = fo * p.fs + fc;
+            const int8_t* wrow = p.fba + ((size_t)f * p.B + b) * K;
+            const int8_t* col = p.ftpa + ((size_t)f * TP + (size_t)w * W) * p.vstride + p.voff;
+            for (int j = 0; j < W; ++j, col += p.vstride) {
+                int re = 0, im = 0;
+                for (int a = 0; a < p.A; ++a) {
+                    const int vr = col[2 * a], vi = col[2 * a + 1];
+                    const int wr = wrow[2 * a], wi = wrow[2 * a + 1];
+                    re += wr * vr - wi * vi;
+                    im += wr * vi + wi * vr;
+                }
+                if (MODE == MODE_SUMS) {
+                    int32_t* dst = static_cast<int32_t*>(p.out) + (((size_t)f * TP + (size_t)w * W + j) * p.B + b) * 2;
+                    dst[0] = re;
+                    dst[1] = im;
+                }
+                P += (long long)re * re + (long long)im * im;
+            }
+        }
+        const size_t o = ((size_t)fo * p.nsamples_out + (size_t)w) * p.B + b;
+        if (MODE == MODE_POWER) static_cast<double*>(p.out)[o] = (double)P;
+        if (MODE == MODE_INT8) static_cast<int8_t*>(p.out)[o] = quantise((double)P, off, sc, p.quant);
+    }
+}
+
+// Whether any weight of beams [b0, b0 + nb) over the fscrunch channels of
+// output channel fo holds a wi = -128, as seen by this lane (lanes take
+// 16-byte pieces t, t + nt, ...).  K = 2A is a multiple of 8: pieces are
+// 16 bytes, or 8 at the end of a row.
+__device__ __forceinline__ bool cb_weights_have_m128(const CbArgs& p, int fo, int b0, int nb, int t, int nt)
+{
+    const int K = 2 * p.A;
+    const int kp = (K + 15) / 16;
+    bool m = false;
+    const int n = p.fs * nb * kp;
+    for (int i = t; i < n; i += nt) {
+        const int fc = i / (nb * kp);
+        const int r = i - fc * nb * kp;
+        const int b = b0 + r / kp;
+        const int k0 = 16 * (r % kp);
+        if (b >= p.B) continue;
+        const int8_t* row = p.fba + ((size_t)(fo * p.fs + fc) * p.B + b) * K;
+        const int2 lo = *reinterpret_cast<const int2*>(row + k0);
+        const int2 hi = k0 + 8 < K ? *reinterpret_cast<const int2*>(row + k0 + 8) : int2{0, 0};
+        m |= raw_has_m128(v4i{lo.x, lo.y, hi.x, hi.y});
+    }
+    return m;
+}
+
 template <int ROWT, int KSTEPS, int MODE, bool AL16>
 __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
 {
@@ -163,6 +275,12 @@ __global__ __launch_bounds__(256) void cb_mfma_k(CbArgs p)
     const long long t_addr = t_valid ? t_out : p.nsamples_out - 1;
     const int tsp = 2 * p.ts;
     const long long TP = 2 * p.nsamples;
+
+    // A wave whose weights hold a wi = -128 takes the exact path.
+    if (wave_any(cb_weights_have_m128(p, fo, beam_base, 8 * ROWT, lane, 64))) {
+        cb_exact_outputs<MODE>(p, fo, beam_base, 8 * ROWT, (long long)tc * 16, (long long)tc * 16 + 16, lane, 64);
+        return;
+    }
 
     long long pw[ROWT][2];
 #pragma unroll
@@ -391,6 +509,12 @@ __global__ __launch_bounds__(256) void cb_mfma32_k(CbArgs p)
     const int tsp = 2 * p.ts;
     const long long TP = 2 * p.nsamples;
 
+    // A wave whose weights hold a wi = -128 takes the exact path.
+    if (wave_any(cb_weights_have_m128(p, fo, beam_base, 16 * RT, lane, 64))) {
+        cb_exact_outputs<MODE>(p, fo, beam_base, 16 * RT, (long long)tc * 32, (long long)tc * 32 + 32, lane, 64);
+        return;
+    }
+
     long long pw[RT][4][2];
 #pragma unroll
     for (int r = 0; r < RT; ++r)
@@ -511,6 +635,13 @@ __global__ __launch_bounds__(256) void cb_lds_k(CbArgs p, int jc)
     const int pieces_per_run = jc * K / PB;   // one output sample's jc columns
     const int npieces = 32 * pieces_per_run;
     const int nchunks = tsp / jc;
+
+    // A work-group any of whose weights hold a wi = -128 takes the exact
+    // path (its waves share the voltage staging, so all of them).
+    if (__syncthreads_or(cb_weights_have_m128(p, fo, bgroup * 4 * 16 * RT, 4 * 16 * RT, tid, 256) ? 1 : 0)) {
+        cb_exact_outputs<MODE>(p, fo, bgroup * 4 * 16 * RT, 4 * 16 * RT, t0, t0 + 32, tid, 256);
+        return;
+    }
 
     long long pw[RT][4][2];
 #pragma unroll
@@ -1094,13 +1225,41 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     int rs = 0;              // ring slot read this step
     int is = NB - St::G;     // next ring slot to fill
 
+    bool m128 = false;
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
+            m128 |= raw_has_m128(w[2 * g][s]);
             w[2 * g + 1][s] = embed_slice(w[2 * g][s], true);
             w[2 * g][s] = embed_slice(w[2 * g][s], false);
         }
+    // A work-group any of whose waves holds a weight wi = -128 computes its
+    // outputs on the exact path instead.  The flags pass through the
+    // thresholds' LDS before the thresholds are written there: written, a
+    // barrier, read by every wave, a barrier.
+    {
+        int* const flags = reinterpret_cast<int*>(thr);
+        const bool wave_m128 = wave_any(m128);   // every lane votes (not inside the lane-0 branch)
+        if (lane == 0) flags[wv] = wave_m128 ? 1 : 0;
+        wait_vm_lgkm0<63>();
+        block_barrier();
+        int any = 0;
+#pragma unroll
+        for (int i = 0; i < St::NW; ++i) any |= flags[i];
+        any = __builtin_amdgcn_readfirstlane(any);
+        wait_vm_lgkm0<63>();
+        block_barrier();
+        if (any) {
+            wait_vm_lgkm0<0>();   // the staging loads land before LDS is released
+            const int tend = (tgroup + 1) * NTS * slot_tiles;
+            cb_exact_outputs<MODE_INT8>(p, fo, bgroup * NWB * 16 * RT, NWB * 16 * RT,
+                                        (long long)tgroup * NTS * slot_tiles * SPT / ta.tps,
+                                        (long long)(tend < ta.ntiles ? tend : ta.ntiles) * SPT / ta.tps, tid,
+                                        (int)blockDim.x);
+            return;
+        }
+    }
 
     // Quantiser: the thresholds are visible after the tile loop's first
     // barrier.
@@ -1467,6 +1626,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void cb_ws_k(CbWsArgs wa)
     int8_t* const ring = lds + wv * NB * St::TILE;
     long long* const thr_all = reinterpret_cast<long long*>(lds + kWsWaves * NB * St::TILE);   // [U][kThr]
     int* const counter = reinterpret_cast<int*>(thr_all + U * kThr);                          // [U]
+    int* const m128_flags = counter + U;                                                      // [kWsWaves]
 
     // Units: id = ((fo * ntr + tr) * nbg + bg).
     auto unit_fo = [&](int gu) { return gu / (wa.ntr * wa.nbg); };
@@ -1475,9 +1635,44 @@ __global__ __launch_bounds__(64 * kWsWaves) void cb_ws_k(CbWsArgs wa)
         thr_all[i] = gu < wa.nunits ? p.thr[(size_t)unit_fo(gu) * kThr + i % kThr] : 0;
     }
     if (tid < U) counter[tid] = 0;
-    __syncthreads();
-
+    // Weights of every unit of the work-group: one holding a wi = -128 sends
+    // the work-group to the exact path (before any staging load is issued).
     const int K = 2 * p.A;
+    {
+        const int kp = K / 16;
+        const int per_unit = 64 * kp;
+        bool m128 = false;
+        for (int i = tid; i < U * per_unit; i += 64 * kWsWaves) {
+            const int gu = unit0 + i / per_unit;
+            const int r = i % per_unit;
+            const int b = (gu % wa.nbg) * 64 + r / kp;
+            if (gu < wa.nunits && b < p.B)
+                m128 |= raw_has_m128(
+                    *reinterpret_cast<const v4i*>(p.fba + ((size_t)unit_fo(gu) * p.B + b) * K + 16 * (r % kp)));
+        }
+        const bool wave_m128 = wave_any(m128);   // every lane votes (not inside the lane-0 branch)
+        if (lane == 0) m128_flags[wv] = wave_m128 ? 1 : 0;
+    }
+    __syncthreads();
+    {
+        int any = 0;
+#pragma unroll
+        for (int i = 0; i < kWsWaves; ++i) any |= m128_flags[i];
+        if (__builtin_amdgcn_readfirstlane(any)) {
+            for (int u = 0; u < U; ++u) {
+                const int gu = unit0 + u;
+                if (gu >= wa.nunits) break;
+                const int tr = (gu / wa.nbg) % wa.ntr;
+                const long long t0 = (long long)tr * wa.tr_chunks * wa.ct;
+                long long t1 = t0 + (long long)wa.tr_chunks * wa.ct;
+                if (t1 > wa.ntiles) t1 = wa.ntiles;
+                cb_exact_outputs<MODE_INT8>(p, unit_fo(gu), (gu % wa.nbg) * 64, 64, t0 * SPT / wa.tps,
+                                            t1 * SPT / wa.tps, tid, (int)blockDim.x);
+            }
+            return;
+        }
+    }
+
     const size_t TP = (size_t)2 * p.nsamples;
     const int kpieces = K / 16;
     const int chunk_tiles = wa.ct;
@@ -1674,7 +1869,8 @@ hipError_t launch_ws_k(CbWsArgs a, hipStream_t s)
             break;
         }
     const long long nblocks = (nunits + a.U - 1) / a.U;
-    const size_t shm = (size_t)kWsWaves * nb * WsStage<KSTEPS>::TILE + (size_t)a.U * (kThr * sizeof(long long) + 4);
+    const size_t shm = (size_t)kWsWaves * nb * WsStage<KSTEPS>::TILE + (size_t)a.U * (kThr * sizeof(long long) + 4) +
+                       kWsWaves * sizeof(int);
     cb_ws_k<KSTEPS, SPT, NSLOT><<<dim3((unsigned)nblocks), 64 * kWsWaves, shm, s>>>(a);
     return hipGetLastError();
 }
@@ -2030,6 +2226,10 @@ extern "C" int fbf_cb_plan_set_levels(fbf_cb_plan* plan, const float* offsets, c
 {
     FBF_REQUIRE(plan && offsets && scales, FBF_ERR_ARG, "fbf_cb_plan_set_levels: null argument");
     const int Fo = plan->d.nchans / plan->d.fscrunch;
+    // A beamform still running on any stream reads the levels and thresholds
+    // being replaced (the library's streams do not wait for the null stream):
+    // it finishes first.
+    FBF_HIP_TRY(hipDeviceSynchronize());
     FBF_HIP_TRY(hipMemcpy(plan->d_levels, offsets, sizeof(float) * Fo, hipMemcpyHostToDevice));
     FBF_HIP_TRY(hipMemcpy(plan->d_levels + Fo, scales, sizeof(float) * Fo, hipMemcpyHostToDevice));
     cb_thresholds_k<<<Fo, 256, 0, nullptr>>>(plan->d_levels, plan->d_levels + Fo, plan->d.quant_mode,

@@ -46,9 +46,13 @@ __global__ __launch_bounds__(256) void ib_k(IbArgs p)
     const int hg = logical / p.nfb;
     if (hg >= p.nhg) return;
 
-    int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    // One word squared is at most 4 * 128^2 = 2^16, so the int32 partials
+    // over one channel's antennas are exact for A < 2^15 (fbf_ib_plan_create
+    // checks it); each channel's partials are then added in int64.
+    long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     const size_t row_bytes = (size_t)S * 4;
     for (int k = 0; k < p.cpb / 4; ++k) {
+        int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
         const int f = fb * p.cpb + row + 4 * k;
         const int8_t* base = p.taftp + ((size_t)hg * p.A * p.F + f) * row_bytes + (size_t)sc * 16;
         const size_t astride = (size_t)p.F * row_bytes;
@@ -59,30 +63,34 @@ __global__ __launch_bounds__(256) void ib_k(IbArgs p)
             for (int u = 0; u < 4; ++u) x[u] = *reinterpret_cast<const int4*>(base + (size_t)(a + u) * astride);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                s0 = sq4(x[u].x, s0);
-                s1 = sq4(x[u].y, s1);
-                s2 = sq4(x[u].z, s2);
-                s3 = sq4(x[u].w, s3);
+                c0 = sq4(x[u].x, c0);
+                c1 = sq4(x[u].y, c1);
+                c2 = sq4(x[u].z, c2);
+                c3 = sq4(x[u].w, c3);
             }
         }
         for (; a < p.A; ++a) {
             const int4 x = *reinterpret_cast<const int4*>(base + (size_t)a * astride);
-            s0 = sq4(x.x, s0);
-            s1 = sq4(x.y, s1);
-            s2 = sq4(x.z, s2);
-            s3 = sq4(x.w, s3);
+            c0 = sq4(x.x, c0);
+            c1 = sq4(x.y, c1);
+            c2 = sq4(x.z, c2);
+            c3 = sq4(x.w, c3);
         }
+        s0 += c0;
+        s1 += c1;
+        s2 += c2;
+        s3 += c3;
     }
 
     // tscrunch: per-sample sums s0..s3 cover samples 4*sc + {0..3}.
     const int nwin = S / p.ts;
     if (p.ts >= 4) {
-        long long v = (long long)s0 + s1 + s2 + s3;
+        long long v = s0 + s1 + s2 + s3;
         for (int m = 1; m < p.ts / 4; m <<= 1) v += __shfl_xor(v, m, 64);
         if ((sc % (p.ts / 4)) == 0) part[row][sc / (p.ts / 4)] = v;
     } else if (p.ts == 2) {
-        part[row][2 * sc] = (long long)s0 + s1;
-        part[row][2 * sc + 1] = (long long)s2 + s3;
+        part[row][2 * sc] = s0 + s1;
+        part[row][2 * sc + 1] = s2 + s3;
     } else {
         part[row][4 * sc] = s0;
         part[row][4 * sc + 1] = s1;
@@ -128,6 +136,10 @@ extern "C" int fbf_ib_plan_create(const fbf_ib_desc* desc, const float* offsets,
     const fbf_ib_desc& d = *desc;
     FBF_REQUIRE(d.nantennas > 0 && d.nchans > 0 && d.tscrunch > 0 && d.fscrunch > 0, FBF_ERR_ARG,
                 "fbf_ib: geometry values must be positive");
+    // Per-channel partial power sums are int32: a word (one sample, both
+    // pols) squares to at most 2^16, so A antennas stay below 2^31 for A < 2^15.
+    FBF_REQUIRE(d.nantennas < 32768, FBF_ERR_UNSUPPORTED,
+                "fbf_ib: nantennas must be below 32768 (int32 per-channel partial power sums)");
     FBF_REQUIRE(d.nsamples_per_heap == 256, FBF_ERR_UNSUPPORTED,
                 "fbf_ib: nsamples_per_heap must be 256 (FBFUSE_NSAMPLES_PER_HEAP)");
     FBF_REQUIRE(256 % d.tscrunch == 0, FBF_ERR_UNSUPPORTED, "fbf_ib: tscrunch must divide 256");
@@ -159,6 +171,8 @@ extern "C" int fbf_ib_plan_set_levels(fbf_ib_plan* plan, const float* offsets, c
 {
     FBF_REQUIRE(plan && offsets && scales, FBF_ERR_ARG, "fbf_ib_plan_set_levels: null argument");
     const int Fo = plan->d.nchans / plan->d.fscrunch;
+    // Work still running on any stream may read the levels being replaced.
+    FBF_HIP_TRY(hipDeviceSynchronize());
     FBF_HIP_TRY(hipMemcpy(plan->d_levels, offsets, sizeof(float) * Fo, hipMemcpyHostToDevice));
     FBF_HIP_TRY(hipMemcpy(plan->d_levels + Fo, scales, sizeof(float) * Fo, hipMemcpyHostToDevice));
     return FBF_OK;

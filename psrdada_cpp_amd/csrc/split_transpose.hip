@@ -55,7 +55,9 @@ __global__ __launch_bounds__(256) void split_transpose_k(
     // Incoherent beam: a 16-byte row load holds 4 samples x 2 pols of one
     // antenna, and one sdot4 of a 4-byte word with itself is the power of
     // both pols of a sample.  This thread sums its 4 samples over the rows it
-    // loads (exact in int32: 4 * 128^2 per word, at most 128 rows).
+    // loads.  A word squares to at most 4 * 128^2 = 2^16, so even the sum of
+    // a sample over all A antennas (below) is exact in int32 for A < 2^15,
+    // which the incoherent-beam plan requires.
     int ibacc[4] = {0, 0, 0, 0};
 
     for (int ab = 0; ab < used_nantennas; ab += kTileA) {

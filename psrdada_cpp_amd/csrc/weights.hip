@@ -11,9 +11,10 @@
 // because f * tau reaches thousands of cycles at GHz x microseconds and FP32
 // would leave ~1e-3 rad of error.  The whole-cycle part is removed in FP64
 // (phase - rint(phase), exact), the reduced argument in [-1/2, 1/2] goes to
-// FP32 sincospif (HIP's device math library), then rounds half away from
-// zero to int8.  One thread per weight, antennas fastest: reads of the
-// delays and writes of the weights are coalesced.
+// FP32 sincospif (HIP's device math library), then rounds to the nearest
+// integer, ties to even, as __float2int_rn does in WeightsManager.cu:70-71.
+// One thread per weight, antennas fastest: reads of the delays and writes of
+// the weights are coalesced.
 #include "fbf_internal.hpp"
 
 namespace fbf {
@@ -41,8 +42,8 @@ __global__ __launch_bounds__(256) void weights_k(const float2* __restrict__ dela
     const float re = 127.0f * c;
     const float im = 127.0f * s;
     char2 w;
-    w.x = (signed char)(int)roundf(re);
-    w.y = (signed char)(int)roundf(im);
+    w.x = (signed char)(int)rintf(re);
+    w.y = (signed char)(int)rintf(im);
     weights[idx] = w;
     if (phase_out) phase_out[idx] = x;
 }

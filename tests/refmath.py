@@ -23,7 +23,7 @@ def ftpa_complex(ftpa, F, T, A, P=2):
     return v[..., 0] + 1j * v[..., 1]
 
 
-def fba_complex(fba, F, B, A, saturate_neg=True):
+def fba_complex(fba, F, B, A):
     w = fba.reshape(F, B, A, 2).astype(np.float64)
     return w[..., 0] + 1j * w[..., 1]
 

@@ -149,25 +149,68 @@ def test_cb_matches_reference_f32_within_reference_tolerance(gpu, oracle):
     got = d_o.download()
     ref = oracle.cb_reference_f32(ftpa, w, F, ts, fs, T, B, A, cbs, cbo, To)
     d = np.abs(got.astype(int) - ref.astype(int))
+    print(f"GPU vs FP32 reference model: {(d > 0).sum()} of {d.size} outputs differ ({(d > 0).mean():.2e})")
     assert d.max() <= 1
-    assert (d > 0).mean() < 0.01
+    # 1 of 32 768 outputs differs for this seed: the FP32 power rounding of
+    # the reference model crosses one quantiser step (the GPU output equals
+    # the exact oracle bit for bit)
+    assert (d > 0).mean() <= 1e-4
 
 
-def test_cb_minus128_weights_follow_documented_saturation(gpu, oracle):
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11])
+@pytest.mark.parametrize("weights", ["sparse", "full_range"])
+@pytest.mark.parametrize("F,T,A,B,ts,fs", [(4, 512, 64, 256, 16, 1), (4, 512, 40, 96, 4, 2),
+                                            (2, 1024, 16, 64, 64, 1)])
+def test_cb_minus128_weights_exact(gpu, oracle, variant, weights, F, T, A, B, ts, fs):
+    """Weight components of -128: the int32 sums follow the reference's exact
+    arithmetic r = wr vr - wi vi (CoherentBeamformer.cu:150-155), so
+    wi = -128 contributes +128 vi, and the power and int8 outputs follow.
+    'sparse': unit-phase weights with a few -128 components (wi alone, wr
+    alone, both, a whole beam), so one launch has waves on the MFMA path and
+    waves on the exact path; 'full_range': uniform int8 weights over
+    [-128, 127], where nearly every wave takes the exact path.  Every kernel
+    variant, sums, power and int8 against the oracle, and the sums also
+    against the numpy complex product."""
     capi = gpu
-    F, T, A, B = 2, 64, 16, 16
-    ftpa = gaussian_int8(800, F * T * 2 * A * 2, 30.0, clip=False)
-    w = unit_weights(801, F * B * A).reshape(F, B, A, 2).copy()
-    w[0, 3, 5, 1] = -128
-    w[1, 7, 0, 1] = -128
-    w[1, 7, 1, 0] = -128
+    ftpa = gaussian_int8(810 + A, F * T * 2 * A * 2, 30.0, clip=False)
+    if weights == "sparse":
+        w = unit_weights(811 + B, F * B * A).reshape(F, B, A, 2).copy()
+        w[0, 3, 5, 1] = -128
+        w[F - 1, B - 7, 0, 1] = -128
+        w[F - 1, B - 7, 1, 0] = -128
+        w[1, B // 2, A - 1, :] = -128
+        w[1, 9, :, 1] = -128
+    else:
+        w = np.random.default_rng(812 + A).integers(-128, 128, F * B * A * 2).astype(np.int8)
     w = w.reshape(-1)
-    off, sc = _levels(oracle, A, 16, 1, F)
-    plan = capi.CbPlan(A, B, F, 16, 1, off, sc)
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, fs)
+    # levels from the power itself (median, spread / 40): few outputs
+    # saturate and one quantiser step is a fraction of a percent of the power
+    off = np.median(p.reshape(F // fs, -1), axis=1).astype(np.float32)
+    sc = (p.reshape(F // fs, -1).std(axis=1) / 40).astype(np.float32)
+    ref_q = oracle.cb_tbtf(p, off, sc, T // ts)
+    # the inputs tell the exact arithmetic from a saturated -(-128) = 127 at int8
+    wsat = w.reshape(F, B, A, 2).astype(np.int64)
+    v = ftpa.reshape(F, T, 2, A, 2).astype(np.int64)
+    nwi = np.where(wsat[..., 1] == -128, 127, -wsat[..., 1])
+    re = np.einsum("ftpa,fba->ftpb", v[..., 0], wsat[..., 0]) + np.einsum("ftpa,fba->ftpb", v[..., 1], nwi)
+    im = np.einsum("ftpa,fba->ftpb", v[..., 1], wsat[..., 0]) + np.einsum("ftpa,fba->ftpb", v[..., 0], wsat[..., 1])
+    p_sat = (re ** 2 + im ** 2).sum(axis=2).reshape(F // fs, fs, T // ts, ts, B).sum(axis=(1, 3)).astype(np.float64)
+    assert np.count_nonzero(oracle.cb_tbtf(p_sat, off, sc, T // ts) != ref_q) > 0
+    plan = capi.CbPlan(A, B, F, ts, fs, off, sc, nsamples_per_heap_out=T // ts)
+    plan.set_variant(variant)
     d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    ref_sums = oracle.cb_sums(ftpa, w, F, T, A, B)
+    np.testing.assert_array_equal(ref_sums, refmath.cb_sums_np(ftpa, w, F, T, A, B))
     d_s = capi.DeviceBuffer(F * T * 2 * B * 2 * 4)
     plan.beam_sums(d_v, d_w, d_s)
-    np.testing.assert_array_equal(d_s.download(np.int32), oracle.cb_sums(ftpa, w, F, T, A, B).reshape(-1))
+    np.testing.assert_array_equal(d_s.download(np.int32), ref_sums.reshape(-1))
+    d_p = capi.DeviceBuffer((F // fs) * (T // ts) * B * 8)
+    plan.beam_power(d_v, d_w, d_p)
+    np.testing.assert_array_equal(d_p.download(np.float64).reshape(F // fs, T // ts, B), p)
+    d_o = capi.DeviceBuffer((F // fs) * (T // ts) * B)
+    plan.beamform(d_v, d_w, d_o)
+    np.testing.assert_array_equal(d_o.download(), ref_q)
 
 
 def test_cb_unit_weights_closed_form(gpu):
@@ -486,3 +529,31 @@ def test_cb_set_levels_applies_to_later_blocks(gpu, oracle, variant):
             plan.set_levels(off, sc)
         plan.beamform(d_v, d_w, d_o)
         np.testing.assert_array_equal(d_o.download(), oracle.cb_tbtf(p, off, sc, T // ts), err_msg=f"levels {k}")
+
+
+def test_cb_set_levels_waits_for_work_in_flight(gpu, oracle):
+    """New levels set while beamforms enqueued on a library stream may still
+    run: those calls finish with the old levels (the update waits for the
+    device), and a call enqueued after it uses the new ones."""
+    capi = gpu
+    F, T, A, B, ts = 16, 8192, 64, 512, 16
+    ftpa = gaussian_int8(95, F * T * 2 * A * 2, 16.0)
+    w = unit_weights(96, F * B * A)
+    off1, sc1 = _levels(oracle, A, ts, 1, F)
+    off2, sc2 = off1 * 1.25, sc1 * 0.5
+    plan = capi.CbPlan(A, B, F, ts, 1, off1, sc1)
+    d_v, d_w = _dev(capi, ftpa), _dev(capi, w)
+    nout = plan.output_bytes(ftpa.nbytes)
+    outs = [capi.DeviceBuffer(nout) for _ in range(4)]
+    s = capi.Stream()
+    for o in outs[:3]:
+        plan.beamform(d_v, d_w, o, stream=s)
+    plan.set_levels(off2, sc2)
+    plan.beamform(d_v, d_w, outs[3], stream=s)
+    s.sync()
+    p = oracle.cb_power(ftpa, w, F, T, A, B, ts, 1)
+    ohs = 8192 // F
+    ref1 = oracle.cb_tbtf(p, off1, sc1, ohs)
+    for k in range(3):
+        np.testing.assert_array_equal(outs[k].download(), ref1, err_msg=f"call {k} (old levels)")
+    np.testing.assert_array_equal(outs[3].download(), oracle.cb_tbtf(p, off2, sc2, ohs))

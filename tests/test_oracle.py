@@ -138,22 +138,31 @@ def test_cb_sums_linear_in_voltages(oracle):
     np.testing.assert_array_equal(s12, s1 + s2)
 
 
-def test_cb_sums_negated_minus128_saturates(oracle):
-    """The Re row carries -wi; -(-128) saturates to 127 (documented semantic)."""
+def test_cb_sums_exact_for_minus128_weights(oracle):
+    """Weight components of -128 (either part): the sums follow the reference's
+    exact int32 arithmetic r = wr vr - wi vi, i = wr vi + wi vr
+    (CoherentBeamformer.cu:150-155), so wi = -128 contributes +128 vi.  A
+    single antenna with w = (-128, -128) and v = (1, -1) gives
+    r = -128 - 128 = -256 and i = 128 - 128 = 0 by hand; the full arrays are
+    checked against the numpy complex product."""
     F, T, A, B = 1, 4, 4, 8
-    ftpa = gaussian_int8(9, F * T * 2 * A * 2, 20.0)
+    ftpa = gaussian_int8(9, F * T * 2 * A * 2, 20.0).reshape(F, T, 2, A, 2).copy()
     w = unit_weights(10, F * B * A).reshape(F, B, A, 2).copy()
     w[0, 1, 2, 1] = -128
-    s = oracle.cb_sums(ftpa, w.reshape(-1), F, T, A, B)
+    w[0, 3, :, :] = -128
+    w[0, 5, 0, 0] = -128
+    ftpa[0, 0, 0, :, 0] = 0
+    ftpa[0, 0, 0, :, 1] = 0
+    ftpa[0, 0, 0, 1, 0] = 1
+    ftpa[0, 0, 0, 1, 1] = -1
+    s = oracle.cb_sums(ftpa.reshape(-1), w.reshape(-1), F, T, A, B)
+    assert s[0, 0, 0, 3, 0] == -256 and s[0, 0, 0, 3, 1] == 0
+    np.testing.assert_array_equal(s, refmath.cb_sums_np(ftpa.reshape(-1), w.reshape(-1), F, T, A, B))
     v = ftpa.reshape(F, T, 2, A, 2).astype(np.int64)
     wr = w[0, 1, :, 0].astype(np.int64)
     wi = w[0, 1, :, 1].astype(np.int64)
-    nwi = -wi
-    nwi[2] = 127
-    re = (v[0, :, :, :, 0] * wr + v[0, :, :, :, 1] * nwi).sum(-1)
-    im = (v[0, :, :, :, 1] * wr + v[0, :, :, :, 0] * wi).sum(-1)
+    re = (v[0, :, :, :, 0] * wr - v[0, :, :, :, 1] * wi).sum(-1)
     np.testing.assert_array_equal(s[0, :, :, 1, 0], re)
-    np.testing.assert_array_equal(s[0, :, :, 1, 1], im)
 
 
 @pytest.mark.parametrize("ts,fs", [(16, 1), (4, 2), (1, 1), (8, 4)])
@@ -214,20 +223,34 @@ def test_cb_tbtf_layout(oracle):
     np.testing.assert_array_equal(got, refmath.tbtf_from_fo_to_b(q, ohs))
 
 
-def test_cb_reference_f32_agrees_within_reference_tolerance(oracle):
+@pytest.mark.parametrize("F,T,A,B,ts,fs,lin,lout,clip,max_rate", [
+    # CoherentBeamformerTester.cu:148-185 (44 antennas, 1024 beams, ts 16,
+    # fs 4, levels 32/32, unclipped N(0, 32)), 8 of its 64 channels and half
+    # its 8192 samples: 39 of 524 288 outputs differ (7.4e-5)
+    (8, 4096, 44, 1024, 16, 4, 32.0, 32.0, False, 1.5e-4),
+    # config-2 levels (64 antennas, 512 beams, ts 16, levels 16/24, clipped
+    # N(0, 16)), 4 channels of a block: 11 of 1 048 576 differ (1.05e-5)
+    (4, 8192, 64, 512, 16, 1, 16.0, 24.0, True, 2e-5),
+])
+def test_cb_reference_f32_agrees_within_reference_tolerance(oracle, F, T, A, B, ts, fs, lin, lout, clip, max_rate):
     """The reference's FP32 host model vs this build's exact-power TRUNC path:
-    within the reference's own +-1 (CoherentBeamformerTester.cu:144)."""
-    F, T, A, B, ts, fs = 8, 256, 44, 32, 16, 4
-    ftpa = gaussian_int8(31, F * T * 2 * A * 2, 32.0, clip=False)
+    within the reference's own +-1 (CoherentBeamformerTester.cu:144), and
+    differing only where the FP32 accumulation of the power lands on the
+    other side of a quantiser step: the rate is measured for these seeded
+    inputs and asserted at about twice that, so a systematic error (a wrong
+    level, a dropped term) fails."""
+    ftpa = gaussian_int8(31, F * T * 2 * A * 2, lin, clip=clip)
     w = unit_weights(32, F * B * A)
-    cbo, cbs, _, _ = oracle.power_levels(32.0, 32.0, A, ts, fs, 64, 16, 2)
+    cbo, cbs, _, _ = oracle.power_levels(lin, lout, A, ts, fs, 64, 16, 2)
     ohs = (T // ts)
     ref = oracle.cb_reference_f32(ftpa, w, F, ts, fs, T, B, A, cbs, cbo, ohs)
     p = oracle.cb_power(ftpa, w, F, T, A, B, ts, fs)
     exact = oracle.cb_tbtf(p, np.full(F // fs, cbo, np.float32), np.full(F // fs, cbs, np.float32), ohs, 1)
     d = np.abs(ref.astype(int) - exact.astype(int))
+    print(f"FP32 reference model vs exact: {(d > 0).sum()} of {d.size} outputs differ "
+          f"({(d > 0).mean():.2e}), max {d.max()}")
     assert d.max() <= 1
-    assert (d > 0).mean() < 0.01
+    assert (d > 0).mean() <= max_rate
 
 
 # ---------------------------------------------------------------------------

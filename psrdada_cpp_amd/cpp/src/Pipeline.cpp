@@ -39,28 +39,31 @@ Pipeline::Pipeline(PipelineConfig const& config, DadaWriteClient& cb_writer, Dad
     const std::size_t heap_group_size =
         _config.total_nantennas() * _config.nchans() * _config.nsamples_per_heap() * _config.npol() * sizeof(char2);
     if (input_data_buffer_size % heap_group_size != 0)
-        throw std::runtime_error("Input DADA buffer is not a multiple of the expected heap group size");
+        throw std::runtime_error("Pipeline: input blocks of " + std::to_string(input_data_buffer_size) +
+                                 " bytes are not a whole number of " + std::to_string(heap_group_size) +
+                                 "-byte heap groups");
     _nheap_groups_per_block = input_data_buffer_size / heap_group_size;
     _nsamples_per_dada_block = _nheap_groups_per_block * _config.nsamples_per_heap();
     if (_nsamples_per_dada_block % _config.cb_tscrunch() != 0 ||
         (_nsamples_per_dada_block / _config.cb_tscrunch()) % _config.cb_nsamples_per_heap() != 0)
-        throw std::runtime_error("Input DADA buffer does not contain an integer multiple of the output heaps");
+        throw std::runtime_error("Pipeline: the spectra of one input block do not fill whole coherent-beam "
+                                 "output heaps");
     _taftp_db.resize(input_data_buffer_size / sizeof(char2), {0, 0});
 
     const std::size_t expected_cb_size = (_config.cb_nbeams() * _nsamples_per_dada_block / _config.cb_tscrunch() *
                                           _config.nchans() / _config.cb_fscrunch());
     if (_cb_writer.data_buffer_size() != expected_cb_size)
-        throw std::runtime_error(std::string("Expected coherent beam output buffer to have a size of ") +
-                                 std::to_string(expected_cb_size) + " bytes, but it instead had a size of " +
-                                 std::to_string(_cb_writer.data_buffer_size()) + " bytes");
+        throw std::runtime_error("Pipeline: one input block gives " + std::to_string(expected_cb_size) +
+                                 " bytes of coherent beams, but the output ring's blocks hold " +
+                                 std::to_string(_cb_writer.data_buffer_size()));
     _tbtf_db.resize(expected_cb_size, 0);
     const std::size_t expected_ib_size =
         (_config.ib_nbeams() * _nsamples_per_dada_block / _config.ib_tscrunch() * _config.nchans() /
          _config.ib_fscrunch());
     if (_ib_writer.data_buffer_size() != expected_ib_size)
-        throw std::runtime_error(std::string("Expected incoherent beam output buffer to have a size of ") +
-                                 std::to_string(expected_ib_size) + " bytes, but it instead had a size of " +
-                                 std::to_string(_ib_writer.data_buffer_size()) + " bytes");
+        throw std::runtime_error("Pipeline: one input block gives " + std::to_string(expected_ib_size) +
+                                 " bytes of incoherent beam, but the output ring's blocks hold " +
+                                 std::to_string(_ib_writer.data_buffer_size()));
     _tf_db.resize(expected_ib_size, 0);
 
     _sample_clock_tick_per_block = 2 * _config.total_nchans() * _nsamples_per_dada_block;   // Pipeline.cu:91
@@ -174,9 +177,9 @@ bool Pipeline::operator()(RawBytes& data)
     FBF_ERROR_CHECK(fbf_stream_synchronize(_d2h_copy_stream));
     _taftp_db.swap();
     if (data.used_bytes() != _taftp_db.a().size() * sizeof(char2))
-        throw std::runtime_error(std::string("Unexpected DADA buffer size, expected ") +
-                                 std::to_string(_taftp_db.a().size() * sizeof(char2)) + " but got " +
-                                 std::to_string(data.used_bytes()));
+        throw std::runtime_error("Pipeline: a block of " + std::to_string(data.used_bytes()) +
+                                 " bytes arrived; the pipeline was set up for " +
+                                 std::to_string(_taftp_db.a().size() * sizeof(char2)));
     FBF_ERROR_CHECK(fbf_memcpy_h2d(_taftp_db.a_ptr(), data.ptr(), data.used_bytes(), _h2d_copy_stream));
     if (_call_count == 1) {
         FBF_ERROR_CHECK(fbf_stream_synchronize(_h2d_copy_stream));

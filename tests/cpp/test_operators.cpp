@@ -16,6 +16,10 @@
 
 #include "fbf_oracle.h"
 #include <thread>
+#include <utility>
+
+#include <sys/wait.h>
+#include <unistd.h>
 
 #include "psrdada_cpp/dada_input_stream.hpp"
 #include "psrdada_cpp/dada_write_client.hpp"
@@ -327,6 +331,129 @@ void dada_ring_write_read_eod()
         threw = true;
     }
     EXPECT(threw);
+}
+
+// A writer in another process fills two observations of a shared ring
+// (SysV segments key and key + 1, as dada_db.cpp:95-110 creates them); this
+// process reads them through DadaInputStream, which attaches a new read
+// client for each observation (detail/dada_input_stream.cpp:29-50).  The
+// writer is forked before anything touches a device.
+struct ObservationRecorder {
+    std::vector<std::string> headers;
+    std::vector<std::pair<int, int>> blocks;   // (observation, block) from the payload
+    std::vector<std::size_t> used;
+    bool payload_ok = true;
+    std::size_t stop_after = 0;
+    void init(RawBytes& h) { headers.emplace_back(h.ptr(), strnlen(h.ptr(), h.used_bytes())); }
+    bool operator()(RawBytes& b)
+    {
+        const int o = b.ptr()[0], i = b.ptr()[1];
+        for (std::size_t k = 2; k < b.used_bytes(); ++k)
+            if (b.ptr()[k] != (char)(o * 31 + i * 7 + k)) payload_ok = false;
+        blocks.emplace_back(o, i);
+        used.push_back(b.used_bytes());
+        return blocks.size() == stop_after;
+    }
+};
+
+void dada_ring_cross_process()
+{
+    DadaDB db(3, 4096, 2, 4096);   // shared placement (the default)
+    db.create();
+    const key_t key = db.key();
+    const int nobs = 2, nblk = 5;
+    std::fflush(stdout);
+    const pid_t pid = fork();
+    if (pid == 0) {
+        int rc = 0;
+        try {
+            MultiLog log("writer");
+            DadaWriteClient writer(key, log);
+            for (int o = 0; o < nobs; ++o) {
+                if (o) writer.reset();   // the next observation
+                auto& h = writer.header_stream().next();
+                std::snprintf(h.ptr(), h.total_bytes(), "OBS %d\nSOURCE J0437-4715\n", o);
+                h.used_bytes(std::strlen(h.ptr()) + 1);
+                writer.header_stream().release();
+                for (int i = 0; i < nblk; ++i) {
+                    auto& b = writer.data_stream().next();
+                    const std::size_t used = b.total_bytes() - 64 * i;
+                    b.ptr()[0] = (char)o;
+                    b.ptr()[1] = (char)i;
+                    for (std::size_t k = 2; k < used; ++k) b.ptr()[k] = (char)(o * 31 + i * 7 + k);
+                    b.used_bytes(used);
+                    writer.data_stream().release(i == nblk - 1);
+                }
+            }
+        } catch (std::exception const& e) {
+            std::fprintf(stderr, "writer: %s\n", e.what());
+            rc = 1;
+        }
+        _exit(rc);
+    }
+    EXPECT(pid > 0);
+    MultiLog log("reader");
+    ObservationRecorder rec;
+    rec.stop_after = (std::size_t)(nobs * nblk);
+    DadaInputStream<ObservationRecorder> stream(key, log, rec);
+    stream.start();
+    int status = 0;
+    EXPECT(waitpid(pid, &status, 0) == pid);
+    EXPECT(WIFEXITED(status) && WEXITSTATUS(status) == 0);
+    EXPECT(rec.headers.size() == (std::size_t)nobs);
+    EXPECT(rec.headers[0].rfind("OBS 0\n", 0) == 0 && rec.headers[1].rfind("OBS 1\n", 0) == 0);
+    EXPECT(rec.blocks.size() == (std::size_t)(nobs * nblk) && rec.payload_ok);
+    for (int o = 0; o < nobs; ++o)
+        for (int i = 0; i < nblk; ++i) {
+            const std::size_t k = (std::size_t)(o * nblk + i);
+            EXPECT(rec.blocks[k] == std::make_pair(o, i) && rec.used[k] == 4096 - 64 * (std::size_t)i);
+        }
+}
+
+// One writer per ring across processes: a second writer in this process is
+// refused while the writer process holds the lock, and takes over a lock
+// left by a writer process that has exited without releasing it.
+void dada_ring_writer_lock_across_processes()
+{
+    DadaDB db(2, 256, 2, 256);
+    db.create();
+    const key_t key = db.key();
+    int to_child[2], to_parent[2];
+    EXPECT(pipe(to_child) == 0 && pipe(to_parent) == 0);
+    std::fflush(stdout);
+    const pid_t pid = fork();
+    if (pid == 0) {
+        MultiLog log("writer");
+        DadaWriteClient* writer = nullptr;
+        char c = 'x';
+        try {
+            writer = new DadaWriteClient(key, log);   // never released: the process exits holding it
+            c = 'L';
+        } catch (...) {
+        }
+        (void)!write(to_parent[1], &c, 1);
+        (void)!read(to_child[0], &c, 1);   // wait until the parent has tried
+        _exit(writer ? 0 : 1);
+    }
+    EXPECT(pid > 0);
+    char c = 0;
+    EXPECT(read(to_parent[0], &c, 1) == 1 && c == 'L');
+    MultiLog log("second");
+    bool refused = false;
+    try {
+        DadaWriteClient second(key, log);
+    } catch (std::runtime_error const&) {
+        refused = true;
+    }
+    EXPECT(refused);
+    (void)!write(to_child[1], "g", 1);
+    int status = 0;
+    EXPECT(waitpid(pid, &status, 0) == pid && WIFEXITED(status) && WEXITSTATUS(status) == 0);
+    DadaWriteClient taker(key, log);   // the exited writer's lock is taken over
+    auto& b = taker.data_stream().next();
+    b.used_bytes(1);
+    taker.data_stream().release();
+    for (int fd : {to_child[0], to_child[1], to_parent[0], to_parent[1]}) close(fd);
 }
 
 // DelayManagerTester.cu:47-67
@@ -662,6 +789,8 @@ int main(int argc, char** argv)
         {"CoherentBeamformer.error_paths_throw", error_paths_throw},
         {"Header.get_set", header_get_set},
         {"DadaRing.write_read_eod", dada_ring_write_read_eod},
+        {"DadaRing.cross_process_two_observations", dada_ring_cross_process},
+        {"DadaRing.writer_lock_across_processes", dada_ring_writer_lock_across_processes},
         {"DelayManagerTester.test_updates", delay_manager_updates},
         {"DelayManagerTester.test_bad_keys", delay_manager_bad_keys},
         {"PipelineTester.simple_run_test", pipeline_simple_run},

@@ -4,11 +4,12 @@
 A step is one DADA block of the single-subarray workload (64 antennas,
 64 channels, 32 heap groups = 8192 spectra = 39.199 ms of data, 512 coherent
 beams, tscrunch 16, plus the incoherent beam), with the voltages already
-resident in HBM:
-    [every 8th block: delay -> weights refresh (FP64 phase)]
-    split-transpose TAFTP -> FTPA
+resident in HBM (three distinct blocks, 403 MB together, rotate so that the
+input is not served from the 256 MB Infinity Cache):
+    delay -> weights at the block's epoch (FP64 phase), every block as
+        Pipeline.cu:164 does
+    split-transpose TAFTP -> FTPA (with the incoherent beam in the same pass)
     coherent beamformer (int8 MFMA, detect, scrunch, quantise) -> TBTF int8
-    incoherent beamformer -> TF int8
 all on one stream.  Prints one JSON line.
 
 value        = x real-time: block duration / wall time per block (K blocks
@@ -90,13 +91,15 @@ def main():
     sub = cfg.subarrays[0]
     A, F, B, T = sub.nantennas, cfg.nchans, sub.nbeams, cfg.nsamples
     seed = 20260101
+    NBLK = 3
     taftp = wl.synth_taftp(cfg, seed)
     delays = wl.synth_delays(B, A, seed + 1, cfg.tau_max_s)
     freqs = cfg.channel_frequencies()
     (cb_off, cb_sc), (ib_off, ib_sc) = cfg.levels(sub)
 
     s = capi.Stream()
-    d_taftp = capi.DeviceBuffer.from_array(taftp, s)
+    d_taftps = [capi.DeviceBuffer.from_array(taftp if i == 0 else wl.synth_taftp(cfg, seed + 100 + i), s)
+                for i in range(NBLK)]
     d_ftpa = capi.DeviceBuffer(cfg.ftpa_bytes(sub))
     d_delays = capi.DeviceBuffer.from_array(delays, s)
     d_w = capi.DeviceBuffer(cfg.weights_bytes(sub))
@@ -114,54 +117,64 @@ def main():
     fused = (A == cfg.total_nantennas and cfg.ib_fscrunch == 1 and 32 % cfg.ib_tscrunch == 0
              and not args.separate_ib)
 
-    def step(k, ev=None):
-        if k % 8 == 0:
-            wp.generate(d_delays, cfg.block_seconds * k, d_w, stream=s)
+    def step(k, ev=None, weights_every=1):
+        d_taftp = d_taftps[k % NBLK]
         if ev:
             ev[0].record(s)
+        if k % weights_every == 0:
+            wp.generate(d_delays, cfg.block_seconds * k, d_w, stream=s)
+        if ev:
+            ev[1].record(s)
         if fused:
             capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
         else:
             capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
         if ev:
-            ev[1].record(s)
+            ev[2].record(s)
         cb.beamform(d_ftpa, d_w, d_cb, stream=s)
         if ev:
-            ev[2].record(s)
+            ev[3].record(s)
         if not fused:
             ib.beamform(d_taftp, d_ib, stream=s)
         if ev:
-            ev[3].record(s)
+            ev[4].record(s)
+
+    def timed_blocks(K, k0, weights_every):
+        """K blocks back to back, events at block boundaries only."""
+        evs = [capi.Event() for _ in range(K + 1)]
+        capi.call("fbf_device_synchronize")
+        t0 = time.perf_counter()
+        evs[0].record(s)
+        for k in range(K):
+            step(k0 + k, weights_every=weights_every)
+            evs[k + 1].record(s)
+        s.sync()
+        wall_ms = (time.perf_counter() - t0) * 1e3
+        return wall_ms, [evs[k].elapsed_ms(evs[k + 1]) for k in range(K)], evs[0].elapsed_ms(evs[K])
 
     for k in range(args.warmup):
         step(k)
     s.sync()
 
-    # Timed region: K blocks back to back, events at block boundaries only.
+    # Timed region: exactly K blocks, weights regenerated every block.
     K = args.steps
-    evs = [capi.Event() for _ in range(K + 1)]
-    capi.call("fbf_device_synchronize")
-    t0 = time.perf_counter()
-    evs[0].record(s)
-    for k in range(K):
-        step(args.warmup + k)
-        evs[k + 1].record(s)
-    s.sync()
-    t1 = time.perf_counter()
-    wall_ms = (t1 - t0) * 1e3
-    block_ms = [evs[k].elapsed_ms(evs[k + 1]) for k in range(K)]
-    dev_total_ms = evs[0].elapsed_ms(evs[K])
+    wall_ms, block_ms, dev_total_ms = timed_blocks(K, args.warmup, 1)
     ms_per_step = wall_ms / K
+
+    # The same K blocks with the weights refreshed every 8th block (the
+    # delay models change slowly; the serving engine refreshes on demand).
+    wall8_ms, _, _ = timed_blocks(K, args.warmup + K, 8)
 
     # Per-kernel durations (separate pass, events around every kernel).
     R = min(50, max(10, K // 4))
-    kev = [[capi.Event() for _ in range(4)] for _ in range(R)]
+    kev = [[capi.Event() for _ in range(5)] for _ in range(R)]
     for r in range(R):
-        step(1 + 8 * r + 1, kev[r])     # k not divisible by 8: no weights refresh inside
+        step(args.warmup + 2 * K + r, kev[r])
     s.sync()
-    st_ms = np.mean([kev[r][0].elapsed_ms(kev[r][1]) for r in range(R)])
-    cb_ms = np.mean([kev[r][1].elapsed_ms(kev[r][2]) for r in range(R)])
-    ib_ms = np.mean([kev[r][2].elapsed_ms(kev[r][3]) for r in range(R)])
+    w_ms = np.mean([kev[r][0].elapsed_ms(kev[r][1]) for r in range(R)])
+    st_ms = np.mean([kev[r][1].elapsed_ms(kev[r][2]) for r in range(R)])
+    cb_ms = np.mean([kev[r][2].elapsed_ms(kev[r][3]) for r in range(R)])
+    ib_ms = np.mean([kev[r][3].elapsed_ms(kev[r][4]) for r in range(R)])
 
     ops = cfg.cb_ops()
     block_s = cfg.block_seconds
@@ -172,7 +185,8 @@ def main():
     st_bytes = 2 * cfg.ftpa_bytes(sub) + (cfg.ib_out_bytes() if fused else 0)
     ib_bytes = cfg.taftp_bytes + cfg.ib_out_bytes()
     st_name = "split_transpose_with_incoherent_beam" if fused else "split_transpose"
-    kernel_ms = {st_name: round(float(st_ms), 4), "coherent_beamformer": round(float(cb_ms), 4)}
+    kernel_ms = {"weights": round(float(w_ms), 4), st_name: round(float(st_ms), 4),
+                 "coherent_beamformer": round(float(cb_ms), 4)}
     kernel_gbs = {st_name: round(st_bytes / (st_ms * 1e-3) / 1e9, 1)}
     if not fused:
         kernel_ms["incoherent_beamformer"] = round(float(ib_ms), 4)
@@ -190,8 +204,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int8 (int32 accumulate, exact int64 detect)",
-        "data": "synthetic: TAFTP voltages round(N(0,16)) clipped to +-127, seeded splitmix64; "
-                "weights from tau ~ U(-1us, 1us) via the FP64-phase generator",
+        "data": "synthetic: TAFTP voltages round(N(0,16)) clipped to +-127, seeded splitmix64, 3 distinct "
+                "blocks (403 MB) rotating; weights from tau ~ U(-1us, 1us) via the FP64-phase generator, "
+                "regenerated every block",
         "config": {
             "workload": cfg.name,
             "nantennas": A, "nchans": F, "nbeams": B, "npol": 2,
@@ -199,8 +214,9 @@ def main():
             "cb_tscrunch": sub.tscrunch, "cb_fscrunch": sub.fscrunch,
             "ib_tscrunch": cfg.ib_tscrunch, "ib_fscrunch": cfg.ib_fscrunch,
             "block_ms": round(block_s * 1e3, 3),
-            "stages": ("weights(every 8 blocks) + split-transpose with IB in the same pass + CB" if fused
-                       else "weights(every 8 blocks) + split-transpose + CB + IB"),
+            "stages": ("weights (every block) + split-transpose with IB in the same pass + CB" if fused
+                       else "weights (every block) + split-transpose + CB + IB"),
+            "input_blocks_rotating": NBLK,
             "device": name.value.decode(),
         },
         "pipeline_int8_tops": round(pipeline_tops, 2),
@@ -210,6 +226,8 @@ def main():
                              "p99": round(percentile(block_ms, 99), 4),
                              "max": round(max(block_ms), 4)},
         "device_ms_per_step": round(dev_total_ms / K, 4),
+        "weights_every_8_blocks": {"ms_per_step": round(wall8_ms / K, 4),
+                                   "x_real_time": round(block_s * 1e3 / (wall8_ms / K), 2)},
         "kernel_ms": kernel_ms,
         "kernel_hbm_gbs": kernel_gbs,
         "roofline": {
@@ -220,15 +238,24 @@ def main():
             "unit": "TOP/s",
             "frac": round(cb_tops / wl.PEAK_INT8_TOPS, 4),
             "traffic": profiled_traffic(),
+            "traffic_source": "profile constant: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of the CB kernels "
+                              "on this workload, scaled by a plain copy's readings, from "
+                              "profiles/cb_64ant_64ch_512beams_hbm_traffic.txt (not measured in this run)",
             "ops_per_launch": ops,
+            "window": "HIP events around fbf_cb_beamform on its stream: cb_tr_k plus the tbtf_k reorder",
         },
     }
 
+    gpu_weights = d_w.download()      # the last block's weights, for the CPU baseline's check
+    gpu_weights_epoch = cfg.block_seconds * (args.warmup + 2 * K + R - 1)
     wp.close()
     cb.close()
     ib.close()
-    for b in (d_taftp, d_ftpa, d_delays, d_w, d_cb, d_ib):
+    for b in (*d_taftps, d_ftpa, d_delays, d_w, d_cb, d_ib):
         b.free()
+    result["pipeline_h2d_inclusive"] = pipeline_leg()
+    result["weights_generation"] = weights_leg(args.warmup)
+    result["config1_block"] = config1_leg(args.warmup)
     if not args.no_mfma_reference:
         ref = mfma_reference_leg()
         if ref is not None:
@@ -241,9 +268,113 @@ def main():
     if not args.no_serving:
         result["serving"] = serving_leg(args.serving_seconds, int(args.hbm_budget_gb * 1e9))
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(cfg, sub, taftp, args.cpu_channels)
+        result["cpu_baseline"] = cpu_baseline(cfg, sub, taftp, args.cpu_channels, delays, freqs, gpu_weights,
+                                              gpu_weights_epoch)
     print(json.dumps(result), flush=True)
     return 0
+
+
+PIPELINE_BENCH = os.path.join(ROOT, "psrdada_cpp_amd", "bin", "fbfuse_pipeline_bench")
+
+
+def pipeline_leg(timed=40, warmup=8):
+    """The drop-in C++ Pipeline at the same geometry with the voltages
+    crossing PCIe every block (pinned host ring blocks, three streams:
+    H2D, processing, D2H), run as a child process of its own (the C++
+    layer's benchmark tool).  None if the tool is not built."""
+    import subprocess
+
+    if not os.path.exists(PIPELINE_BENCH):
+        print("pipeline leg skipped: fbfuse_pipeline_bench not built", file=sys.stderr)
+        return None
+    try:
+        r = subprocess.run([PIPELINE_BENCH, str(timed), str(warmup)], capture_output=True, text=True, timeout=300)
+        lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+        if r.returncode != 0 or not lines:
+            print(f"pipeline leg failed: exit {r.returncode}: {r.stderr[-500:]}", file=sys.stderr)
+            return None
+        return json.loads(lines[-1])
+    except (OSError, subprocess.SubprocessError, ValueError) as ex:
+        print(f"pipeline leg skipped: {type(ex).__name__}: {ex}", file=sys.stderr)
+        return None
+
+
+def weights_leg(warmup, reps=50):
+    """Delay -> weights for 64 antennas x 1024 beams x 64 channels (tau0 ~
+    U(-2us, 2us), rate ~ U(-1e-9, 1e-9)), epochs spread over 600 s: HIP
+    events around `reps` launches on one stream.  The accuracy against the
+    FP64 oracle is checked in the CPU baseline."""
+    from psrdada_cpp_amd import capi, workloads as wl
+
+    A, B, F = 64, 1024, 64
+    freqs = capi.channel_frequencies_f64(wl.L_BAND_CENTRE_HZ, F * wl.CHANNEL_BW_HZ, F)
+    d = wl.synth_delays(B, A, 4242, 2e-6, rate_max=1e-9)
+    s = capi.Stream()
+    d_d = capi.DeviceBuffer.from_array(d, s)
+    d_w = capi.DeviceBuffer(F * B * A * 2)
+    wp = capi.WeightsPlan(A, B, F, freqs)
+    for k in range(warmup):
+        wp.generate(d_d, 600.0 * k / reps, d_w, stream=s)
+    s.sync()
+    e0, e1 = capi.Event(), capi.Event()
+    e0.record(s)
+    for k in range(reps):
+        wp.generate(d_d, 600.0 * k / reps, d_w, stream=s)
+    e1.record(s)
+    s.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    n = F * B * A
+    block_ms = wl.config2().block_seconds * 1e3
+    wp.close()
+    d_d.free()
+    d_w.free()
+    return {"workload": "64 ant x 1024 beams x 64 ch, tau0 ~ U(-2us, 2us), rate ~ U(-1e-9, 1e-9), epochs over 600 s",
+            "ms_per_launch": round(ms, 4), "weights_per_s": float(f"{n / (ms * 1e-3):.4g}"),
+            "hbm_gbs": round((n * 2 + B * A * 8) / (ms * 1e-3) / 1e9, 1),
+            "refresh_every_8_blocks_fraction_of_real_time": float(f"{ms / (8 * block_ms):.3g}")}
+
+
+def config1_leg(warmup, reps=200):
+    """The CPU plumbing case on the GPU: 16 antennas, 16 channels, 8 heap
+    groups (2048 spectra = 9.8 ms), 32 beams, tscrunch 16: weights,
+    split-transpose + incoherent beam, coherent beam per block, HIP events
+    around `reps` blocks."""
+    from psrdada_cpp_amd import capi, workloads as wl
+
+    cfg = wl.config1()
+    sub = cfg.subarrays[0]
+    A, F, B = sub.nantennas, cfg.nchans, sub.nbeams
+    (cb_off, cb_sc), (ib_off, ib_sc) = cfg.levels(sub)
+    s = capi.Stream()
+    d_t = capi.DeviceBuffer.from_array(wl.synth_taftp(cfg, 11), s)
+    d_d = capi.DeviceBuffer.from_array(wl.synth_delays(B, A, 12, cfg.tau_max_s), s)
+    d_f = capi.DeviceBuffer(cfg.ftpa_bytes(sub))
+    d_w = capi.DeviceBuffer(cfg.weights_bytes(sub))
+    wp = capi.WeightsPlan(A, B, F, cfg.channel_frequencies())
+    cb = capi.CbPlan(A, B, F, sub.tscrunch, sub.fscrunch, cb_off, cb_sc, nsamples_per_heap_out=cfg.cb_ohs(sub))
+    ib = capi.IbPlan(A, F, cfg.ib_tscrunch, cfg.ib_fscrunch, ib_off, ib_sc)
+    d_cb = capi.DeviceBuffer(cfg.cb_out_bytes(sub))
+    d_ib = capi.DeviceBuffer(cfg.ib_out_bytes())
+
+    def block(k):
+        wp.generate(d_d, cfg.block_seconds * k, d_w, stream=s)
+        capi.split_transpose_ib(d_t, d_f, F, ib, d_ib, stream=s)
+        cb.beamform(d_f, d_w, d_cb, stream=s)
+
+    for k in range(warmup):
+        block(k)
+    s.sync()
+    e0, e1 = capi.Event(), capi.Event()
+    e0.record(s)
+    for k in range(reps):
+        block(k)
+    e1.record(s)
+    s.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    for o in (wp, cb, ib):
+        o.close()
+    return {"workload": cfg.name, "device_ms_per_block": round(ms, 4),
+            "x_real_time": round(cfg.block_seconds * 1e3 / ms, 1), "block_ms": round(cfg.block_seconds * 1e3, 3)}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cb_64ant_64ch_512beams_hbm_traffic.txt")
@@ -398,8 +529,15 @@ def batch_leg(K, warmup):
         t_tune = time.perf_counter() - t_tune
         s = capi.Stream()
         d_taftp = capi.DeviceBuffer.from_array(wl.synth_taftp(cfg, 31 + ts), s)
-        ws = [capi.DeviceBuffer.from_array(capi.synth_gaussian_int8(40 + i, 60.0, batch.weights_bytes(i)), s)
-              for i in range(len(subs))]
+        ws = []
+        for i, sb in enumerate(cfg.subarrays):
+            wpi = capi.WeightsPlan(sb.nantennas, sb.nbeams, cfg.nchans, cfg.channel_frequencies())
+            d_wi = capi.DeviceBuffer(batch.weights_bytes(i))
+            wpi.generate(capi.DeviceBuffer.from_array(wl.synth_delays(sb.nbeams, sb.nantennas, 40 + i,
+                                                                      cfg.tau_max_s), s), 0.0, d_wi, stream=s)
+            s.sync()
+            wpi.close()
+            ws.append(d_wi)
         outs = [capi.DeviceBuffer(batch.output_bytes(i)) for i in range(len(subs) + 1)]
         for _ in range(warmup):
             batch.run(d_taftp, ws, outs, use_graph=True, stream=s)
@@ -431,7 +569,10 @@ def batch_leg(K, warmup):
 def serving_leg(seconds, budget):
     """Four feeds (64/768/64, 32/256/64, 16/128/32, 8/64/16 ant/beams/ch),
     Poisson arrivals at 1.5x real time per feed from pinned host memory, then
-    closed-loop saturation with host and with device-resident input."""
+    closed-loop saturation with host and with device-resident input for 2 s
+    each; and the 64 ant/512 beams/64 ch geometry as a feed of its own,
+    device-resident, closed loop for 2.5 s (throughput and p99 through the
+    engine).  One untimed warm-up block per feed precedes each run."""
     from psrdada_cpp_amd import serving
 
     feeds = serving.four_subarray_feeds(32)
@@ -441,21 +582,32 @@ def serving_leg(seconds, budget):
             "workload": "4 feeds 64/768/64, 32/256/64, 16/128/32, 8/64/16 (ant/beams/ch), 32 heap groups, "
                         "ts 16 + IB; continuous batching over 8 streams",
             "poisson_1.5x": serving.run_poisson(srv, fids, feeds, 1.5, seconds),
-            "saturation_host_input": serving.run_saturation(srv, fids, feeds, 24),
-            "saturation_device_input": serving.run_saturation(srv, fids, feeds, 64, device_input=True),
+            "saturation_host_input": serving.run_saturation(srv, fids, feeds, seconds=2.0),
+            "saturation_device_input": serving.run_saturation(srv, fids, feeds, device_input=True, seconds=2.0),
         }
     finally:
         srv.close()
     out["block_ms"] = round(feeds[0].block_seconds * 1e3, 3)
+    one = [serving.FeedSpec(64, 512, 64, 32)]
+    srv, fids = serving.make_server(one, budget, nslots=8)
+    try:
+        r = serving.run_saturation(srv, fids, one, device_input=True, seconds=2.5, nout=8)
+        r["workload"] = "one feed 64 ant/512 beams/64 ch, 32 heap groups, ts 16 + IB, device-resident input"
+        out["feed_64ant_512beams_64ch_device_input"] = r
+    finally:
+        srv.close()
     return out
 
 
-def cpu_baseline(cfg, sub, taftp, nch):
+def cpu_baseline(cfg, sub, taftp, nch, delays, freqs, gpu_weights, gpu_weights_epoch):
     """The CPU oracle on a bounded sample of the same block: split-transpose
     and IB over the whole block, CB (exact power + quantiser) over `nch` of the
-    64 channels, scaled linearly in F to a full block."""
+    64 channels, scaled linearly in F to a full block.  Also the CPU side of
+    config 1 (a whole block) and config 4 (the FP64 weights generator), and,
+    as the checker, the GPU weights of the last benchmarked block and of the
+    config-4 workload against the FP64 oracle."""
     import oracle
-    from psrdada_cpp_amd import capi
+    from psrdada_cpp_amd import capi, workloads as wl
 
     try:
         cores = len(os.sched_getaffinity(0))
@@ -486,6 +638,32 @@ def cpu_baseline(cfg, sub, taftp, nch):
     oracle.cb_tbtf(p1, np.ones(n1, np.float32), np.ones(n1, np.float32), T // sub.tscrunch)
     u2 = time.perf_counter()
     block_1core_s = (t1 - t0) + (u1 - u0) + (u2 - u1) * (F / n1)
+    # Config 1: a whole block on the CPU (split-transpose, IB, CB + quantiser).
+    c1 = wl.config1()
+    s1 = c1.subarrays[0]
+    t1_in = wl.synth_taftp(c1, 11)
+    w1 = oracle.weights_f64(wl.synth_delays(s1.nbeams, s1.nantennas, 12, c1.tau_max_s), c1.channel_frequencies(),
+                            0.0, s1.nantennas, s1.nbeams, c1.nchans)
+    (o1, c1s), (io1, is1) = c1.levels(s1)
+    v0 = time.perf_counter()
+    f1 = oracle.split_transpose(t1_in, c1.nheap_groups, s1.nantennas, s1.nantennas, 0, c1.nchans)
+    oracle.ib_tf(oracle.ib_power(t1_in, c1.nchans, c1.ib_tscrunch, 1, c1.nheap_groups, s1.nantennas, nthreads=cores),
+                 io1, is1)
+    oracle.cb_tbtf(oracle.cb_power(f1, w1, c1.nchans, c1.nsamples, s1.nantennas, s1.nbeams, s1.tscrunch, 1,
+                                   nthreads=cores), o1, c1s, c1.cb_ohs(s1))
+    c1_ms = (time.perf_counter() - v0) * 1e3
+    # Config 4: the FP64 weights generator on the CPU (one thread), and the
+    # GPU's weights against it.
+    A4, B4, F4 = 64, 1024, 64
+    fr4 = capi.channel_frequencies_f64(wl.L_BAND_CENTRE_HZ, F4 * wl.CHANNEL_BW_HZ, F4)
+    d4 = wl.synth_delays(B4, A4, 4242, 2e-6, rate_max=1e-9)
+    x0 = time.perf_counter()
+    w4_ref, frac4 = oracle.weights_f64(d4, fr4, 300.0, A4, B4, F4, want_phase=True)
+    w4_s = time.perf_counter() - x0
+    w4_gpu, ph4_gpu = _gpu_weights(d4, fr4, 300.0, A4, B4, F4)
+    dw4 = np.abs(w4_gpu.astype(int) - w4_ref.astype(int))
+    wref2 = oracle.weights_f64(delays, freqs, gpu_weights_epoch, sub.nantennas, sub.nbeams, F)
+    dw2 = np.abs(gpu_weights.astype(int) - wref2.astype(int))
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -507,7 +685,34 @@ def cpu_baseline(cfg, sub, taftp, nch):
         "one_core": {"value": round(cfg.block_seconds / block_1core_s, 5), "unit": "x real-time",
                      "sample": f"split-transpose and IB over the whole block, CB over {n1} channels scaled "
                                f"x{F // n1}, 1 thread; extrapolated block time {block_1core_s:.1f} s"},
+        "config1_block": {"ms_per_block": round(c1_ms, 3), "x_real_time": round(c1.block_seconds * 1e3 / c1_ms, 3),
+                          "threads": cores, "sample": "one whole config-1 block (16 ant, 16 ch, 2048 spectra, "
+                                                      "32 beams): split-transpose, IB, CB and quantiser"},
+        "config4_weights": {"cpu_fp64_seconds": round(w4_s, 3), "cpu_weights_per_s": float(f"{A4 * B4 * F4 / w4_s:.4g}"),
+                            "threads": 1,
+                            "gpu_vs_fp64_oracle": {
+                                "max_phase_error_rad": float(f"{np.abs(ph4_gpu.astype(np.float64) - frac4).max() * 2 * np.pi:.3g}"),
+                                "fraction_1_lsb": float(f"{(dw4 > 0).mean():.3g}"), "max_lsb": int(dw4.max()),
+                                "epoch_s": 300.0},
+                            "config2_last_block_fraction_1_lsb": float(f"{(dw2 > 0).mean():.3g}"),
+                            "config2_last_block_max_lsb": int(dw2.max())},
     }
+
+
+def _gpu_weights(delays, freqs, epoch, A, B, F):
+    """GPU weights and reduced phases for the CPU baseline's check."""
+    from psrdada_cpp_amd import capi
+
+    wp = capi.WeightsPlan(A, B, F, freqs)
+    d_d = capi.DeviceBuffer.from_array(delays)
+    d_w = capi.DeviceBuffer(F * B * A * 2)
+    d_ph = capi.DeviceBuffer(F * B * A * 4)
+    wp.generate(d_d, epoch, d_w, d_ph)
+    out = d_w.download(), d_ph.download(np.float32)
+    wp.close()
+    for b in (d_d, d_w, d_ph):
+        b.free()
+    return out
 
 
 if __name__ == "__main__":

@@ -22,6 +22,7 @@ CPP = os.path.join(PKG, "cpp")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libfbfuse_hip.so")
 CPP_LIB = os.path.join(PKG, "libfbfuse_cpp.so")
+BIN = os.path.join(PKG, "bin")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -108,6 +109,18 @@ def build_cpp(verbose=False, jobs=8):
               "-Wl,-rpath,$ORIGIN/../../psrdada_cpp_amd", "-Wl,-rpath,$ORIGIN/../../oracle"], verbose)
         os.replace(tmpx, exe)
         outs.append(exe)
+    # Command-line tools over the C++ layer (cpp/tools/<name>.cpp ->
+    # bin/fbfuse_<name>), e.g. the Pipeline benchmark that bench.py runs.
+    tool_dir = os.path.join(CPP, "tools")
+    if os.path.isdir(tool_dir):
+        os.makedirs(BIN, exist_ok=True)
+        for s in _sources(tool_dir, (".cpp",)):
+            exe = os.path.join(BIN, "fbfuse_" + os.path.splitext(os.path.basename(s))[0])
+            tmpx = exe + ".tmp"
+            _run([*cxx, *inc, s, "-o", tmpx, "-L", PKG, "-lfbfuse_cpp", "-lfbfuse_hip",
+                  "-Wl,-rpath,$ORIGIN/.."], verbose)
+            os.replace(tmpx, exe)
+            outs.append(exe)
     return CPP_LIB, outs
 
 

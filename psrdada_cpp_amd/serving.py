@@ -112,12 +112,21 @@ class _Buffers:
         self.sem[k].release()
 
 
-def _summary(srv, feeds, counts, wall_s, block_lat):
+def _pct(x):
+    x = np.asarray(x) if len(x) else np.zeros(1)
+    return {"p50": round(float(np.percentile(x, 50)), 3), "p99": round(float(np.percentile(x, 99)), 3),
+            "max": round(float(x.max()), 3)}
+
+
+def _summary(srv, feeds, counts, wall_s, block_lat, feed_lat=None):
     st = srv.stats()
     data_s = sum(c * fs.block_seconds for c, fs in zip(counts, feeds))
     ops = sum(c * fs.cb_ops for c, fs in zip(counts, feeds))
     nbytes = sum(c * fs.block_bytes for c, fs in zip(counts, feeds))
     lat = np.asarray(block_lat) if block_lat else np.zeros(1)
+    per_feed = {}
+    if feed_lat is not None:
+        per_feed = {f"{fs.nantennas}/{fs.nbeams}/{fs.nchans}": _pct(fl) for fs, fl in zip(feeds, feed_lat)}
     return {
         "blocks": int(sum(counts)),
         "blocks_per_feed": [int(c) for c in counts],
@@ -131,6 +140,7 @@ def _summary(srv, feeds, counts, wall_s, block_lat):
                        "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(lat.max()), 3),
                        "mean": round(float(lat.mean()), 3)},
+        "latency_ms_per_feed": per_feed,
         "engine_p99_ms": round(st["p99_ms"], 3),
         "failed": int(st["failed"]),
         "hbm_budget": int(st["hbm_budget"]),
@@ -138,15 +148,24 @@ def _summary(srv, feeds, counts, wall_s, block_lat):
     }
 
 
-def _collector(srv, bufs, owner, n, lat, counts, errors):
-    """Polls n completions, returns their output buffers to the pool."""
+def _collector(srv, bufs, owner, n, lat, counts, errors, feed_lat=None):
+    """Polls n completions (n None: until the owner map holds a None entry
+    and is otherwise empty), returns their output buffers to the pool."""
     got = 0
+    last = time.perf_counter()
     try:
-        while got < n:
-            r = srv.poll(timeout_ms=30000)
+        while n is None or got < n:
+            if n is None and owner.get(-1, 0) is None and len(owner) == 1:
+                return
+            # Short polls when the count is open, so that the end marker is
+            # seen; 30 s without a completion is a failure either way.
+            r = srv.poll(timeout_ms=30000 if n is not None else 50)
             if r is None:
+                if n is None and time.perf_counter() - last < 30.0:
+                    continue
                 errors.append("no completion within 30 s")
                 return
+            last = time.perf_counter()
             tag, status, ms = r
             k, o = owner.pop(tag)
             bufs.give(k, o)
@@ -155,9 +174,24 @@ def _collector(srv, bufs, owner, n, lat, counts, errors):
             else:
                 lat.append(ms)
                 counts[k] += 1
+                if feed_lat is not None:
+                    feed_lat[k].append(ms)
             got += 1
     except Exception as e:  # reported by the caller after join()
         errors.append(repr(e))
+
+
+def warm_up(srv, fids, feeds, seed=3, device_input=False):
+    """One untimed block of every feed, so that code-object loading and
+    first-launch costs are not counted as serving latency."""
+    bufs = _Buffers(srv, fids, feeds, seed, 1, 1, device_input)
+    for k, fid in enumerate(fids):
+        o = bufs.take(k)
+        srv.submit(fid, bufs.inputs[k][0], o[0], o[1], tag=10**9 + k, on_device=device_input)
+    for _ in fids:
+        r = srv.poll(timeout_ms=30000)
+        if r is None or r[1] != 0:
+            raise RuntimeError(f"warm-up block failed: {r}")
 
 
 def run_poisson(srv, fids, feeds, load_x=1.5, duration_s=10.0, seed=7, device_input=False, nout=8):
@@ -172,10 +206,12 @@ def run_poisson(srv, fids, feeds, load_x=1.5, duration_s=10.0, seed=7, device_in
             arrivals.append((t, k))
             t += rng.exponential(1.0 / rate)
     arrivals.sort()
+    warm_up(srv, fids, feeds, seed, device_input)
     bufs = _Buffers(srv, fids, feeds, seed, 2, nout, device_input)
     owner, lat, counts, errors = {}, [], [0] * len(feeds), []
+    feed_lat = [[] for _ in feeds]
     srv.reset_stats()
-    col = threading.Thread(target=_collector, args=(srv, bufs, owner, len(arrivals), lat, counts, errors))
+    col = threading.Thread(target=_collector, args=(srv, bufs, owner, len(arrivals), lat, counts, errors, feed_lat))
     col.start()
     t0 = time.perf_counter()
     late = 0
@@ -192,32 +228,40 @@ def run_poisson(srv, fids, feeds, load_x=1.5, duration_s=10.0, seed=7, device_in
     wall = time.perf_counter() - t0
     if errors:
         raise RuntimeError("; ".join(errors))
-    out = _summary(srv, feeds, counts, wall, lat)
+    out = _summary(srv, feeds, counts, wall, lat, feed_lat)
     out.update({"offered_load_x": load_x, "arrivals": "poisson", "late_submissions": late,
                 "input": "device" if device_input else "pinned host (PCIe-inclusive)"})
     return out
 
 
-def run_saturation(srv, fids, feeds, nblocks_per_feed=50, device_input=False, seed=11, nout=4):
+def run_saturation(srv, fids, feeds, nblocks_per_feed=50, device_input=False, seed=11, nout=4, seconds=None):
     """Closed loop: blocks of all feeds round-robin, at most `nout` outstanding
-    per feed, submitted as fast as the engine completes them."""
+    per feed, submitted as fast as the engine completes them; either
+    `nblocks_per_feed` blocks of each feed or, with `seconds`, for that long
+    (after one untimed warm-up block per feed)."""
+    warm_up(srv, fids, feeds, seed, device_input)
     bufs = _Buffers(srv, fids, feeds, seed, 2, nout, device_input)
     owner, lat, counts, errors = {}, [], [0] * len(feeds), []
-    total = nblocks_per_feed * len(feeds)
+    feed_lat = [[] for _ in feeds]
+    total = None if seconds else nblocks_per_feed * len(feeds)
     srv.reset_stats()
-    col = threading.Thread(target=_collector, args=(srv, bufs, owner, total, lat, counts, errors))
+    col = threading.Thread(target=_collector, args=(srv, bufs, owner, total, lat, counts, errors, feed_lat))
     t0 = time.perf_counter()
     col.start()
-    for tag in range(total):
+    tag = 0
+    while (tag < total) if total is not None else (time.perf_counter() - t0 < seconds):
         k = tag % len(feeds)
         o = bufs.take(k)
         owner[tag] = (k, o)
         srv.submit(fids[k], bufs.inputs[k][tag % 2], o[0], o[1], tag=tag, on_device=device_input)
+        tag += 1
+    if total is None:
+        owner[-1] = None   # no more submissions: the collector ends once the rest completed
     col.join()
     wall = time.perf_counter() - t0
     if errors:
         raise RuntimeError("; ".join(errors))
-    out = _summary(srv, feeds, counts, wall, lat)
+    out = _summary(srv, feeds, counts, wall, lat, feed_lat)
     out.update({"arrivals": "closed loop", "outstanding_per_feed": nout,
                 "input": "device" if device_input else "pinned host (PCIe-inclusive)"})
     return out

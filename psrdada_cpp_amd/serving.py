@@ -185,13 +185,19 @@ def warm_up(srv, fids, feeds, seed=3, device_input=False):
     """One untimed block of every feed, so that code-object loading and
     first-launch costs are not counted as serving latency."""
     bufs = _Buffers(srv, fids, feeds, seed, 1, 1, device_input)
+    # The output buffers of every block in flight stay referenced (the
+    # engine copies into them) until its completion has been polled.
+    outs = []
     for k, fid in enumerate(fids):
-        o = bufs.take(k)
-        srv.submit(fid, bufs.inputs[k][0], o[0], o[1], tag=10**9 + k, on_device=device_input)
+        outs.append(bufs.take(k))
+        srv.submit(fid, bufs.inputs[k][0], outs[k][0], outs[k][1], tag=10**9 + k, on_device=device_input)
     for _ in fids:
         r = srv.poll(timeout_ms=30000)
         if r is None or r[1] != 0:
+            # an unfinished block may still write its outputs: wait for all
+            srv.drain()
             raise RuntimeError(f"warm-up block failed: {r}")
+    del outs
 
 
 def run_poisson(srv, fids, feeds, load_x=1.5, duration_s=10.0, seed=7, device_input=False, nout=8):
@@ -227,6 +233,7 @@ def run_poisson(srv, fids, feeds, load_x=1.5, duration_s=10.0, seed=7, device_in
     col.join()
     wall = time.perf_counter() - t0
     if errors:
+        srv.drain()   # blocks still running write into the pools' buffers
         raise RuntimeError("; ".join(errors))
     out = _summary(srv, feeds, counts, wall, lat, feed_lat)
     out.update({"offered_load_x": load_x, "arrivals": "poisson", "late_submissions": late,
@@ -260,6 +267,7 @@ def run_saturation(srv, fids, feeds, nblocks_per_feed=50, device_input=False, se
     col.join()
     wall = time.perf_counter() - t0
     if errors:
+        srv.drain()   # blocks still running write into the pools' buffers
         raise RuntimeError("; ".join(errors))
     out = _summary(srv, feeds, counts, wall, lat, feed_lat)
     out.update({"arrivals": "closed loop", "outstanding_per_feed": nout,

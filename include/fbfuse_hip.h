@@ -157,6 +157,16 @@ int fbf_cb_beamform_ws(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
                        const void* fba_weights, size_t weights_bytes,
                        void* tbtf, size_t tbtf_bytes, void* workspace, size_t workspace_bytes,
                        fbf_stream_t stream);
+/* As fbf_cb_beamform, allocating nothing: the int8 beams are staged in a
+ * fixed device array of the library (32 MiB per GPU, held until the process
+ * ends) instead of a buffer the plan allocates.  No counterpart in the
+ * reference; for callers that cannot allocate at call time (code under
+ * stream capture, tight serving loops).  A staged output (the size of the
+ * TBTF output) above 32 MiB returns FBF_ERR_SIZE.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int fbf_cb_beamform_noalloc(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
+                            const void* fba_weights, size_t weights_bytes,
+                            void* tbtf, size_t tbtf_bytes, fbf_stream_t stream);
 /* Sizes the plan's staging buffer for blocks of nsamples samples (call before
  * capturing a stream that runs the plan). */
 int fbf_cb_plan_reserve(fbf_cb_plan* plan, long long nsamples);

@@ -122,6 +122,7 @@ _SIGS = {
     "fbf_batch_autotune": [vp, i32],
     "fbf_batch_run": [vp, vp, sz, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), i32, vp],
     "fbf_cb_beamform_ws": [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp],
+    "fbf_cb_beamform_noalloc": [vp, vp, sz, vp, sz, vp, sz, vp],
     "fbf_split_transpose_ib": [vp, sz, vp, sz, i32, i32, vp, vp, sz, vp],
     "fbf_tuse_transpose": [vp, sz, vp, sz, i32, i32, i32, i32, i32, i32, i32, vp],
     "fbf_feng_to_dada": [vp, vp, sz, i32, vp],

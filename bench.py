@@ -111,7 +111,14 @@ def main():
                 for i in range(NBLK)]
     d_ftpa = capi.DeviceBuffer(cfg.ftpa_bytes(sub))
     d_delays = capi.DeviceBuffer.from_array(delays, s)
-    d_w = capi.DeviceBuffer(cfg.weights_bytes(sub))
+    # Two weights buffers: block k's weights are generated on a second stream
+    # while the split-transpose of block k runs, into the buffer the CB of
+    # block k - 2 read (its completion event orders the reuse).
+    s_w = capi.Stream()
+    d_ws = [capi.DeviceBuffer(cfg.weights_bytes(sub)) for _ in range(2)]
+    w_ready = [capi.Event() for _ in range(2)]
+    cb_done = [capi.Event() for _ in range(2)]
+    wstate = {"gen": 0, "cur": 0, "cb_recorded": [False, False]}
     wp = capi.WeightsPlan(A, B, F, freqs)
     cb = capi.CbPlan(A, B, F, sub.tscrunch, sub.fscrunch, cb_off, cb_sc,
                      nsamples_per_heap_out=cfg.cb_ohs(sub))
@@ -127,20 +134,37 @@ def main():
              and not args.separate_ib)
 
     def step(k, ev=None, weights_every=1):
+        """One block.  With `ev` (five events) every kernel runs on stream s
+        in turn, for per-kernel durations; otherwise the weights go on s_w
+        beside the split-transpose."""
         d_taftp = d_taftps[k % NBLK]
+        new_w = k % weights_every == 0
+        if new_w:
+            b = wstate["gen"] % 2
+            wstate["gen"] += 1
+            wstate["cur"] = b
+        d_w = d_ws[wstate["cur"]]
         if ev:
             ev[0].record(s)
-        if k % weights_every == 0:
-            wp.generate(d_delays, cfg.block_seconds * k, d_w, stream=s)
-        if ev:
+            if new_w:
+                wp.generate(d_delays, cfg.block_seconds * k, d_w, stream=s)
             ev[1].record(s)
+        elif new_w:
+            if wstate["cb_recorded"][b]:
+                s_w.wait(cb_done[b])
+            wp.generate(d_delays, cfg.block_seconds * k, d_w, stream=s_w)
+            w_ready[b].record(s_w)
         if fused:
             capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
         else:
             capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
         if ev:
             ev[2].record(s)
+        elif new_w:
+            s.wait(w_ready[b])
         cb.beamform(d_ftpa, d_w, d_cb, stream=s)
+        cb_done[wstate["cur"]].record(s)
+        wstate["cb_recorded"][wstate["cur"]] = True
         if ev:
             ev[3].record(s)
         if not fused:
@@ -236,8 +260,10 @@ def main():
             "cb_tscrunch": sub.tscrunch, "cb_fscrunch": sub.fscrunch,
             "ib_tscrunch": cfg.ib_tscrunch, "ib_fscrunch": cfg.ib_fscrunch,
             "block_ms": round(block_s * 1e3, 3),
-            "stages": ("weights (every block) + split-transpose with IB in the same pass + CB" if fused
-                       else "weights (every block) + split-transpose + CB + IB"),
+            "stages": ("weights (every block, on a second stream beside the split-transpose) + split-transpose "
+                       "with IB in the same pass + CB" if fused
+                       else "weights (every block, on a second stream beside the split-transpose) + "
+                            "split-transpose + CB + IB"),
             "input_blocks_rotating": NBLK,
             "device": name.value.decode(),
         },
@@ -271,12 +297,12 @@ def main():
         },
     }
 
-    gpu_weights = d_w.download()      # the last block's weights, for the CPU baseline's check
+    gpu_weights = d_ws[wstate["cur"]].download()   # the last block's weights, for the CPU baseline's check
     gpu_weights_epoch = cfg.block_seconds * (args.warmup + 2 * K + R - 1)
     wp.close()
     cb.close()
     ib.close()
-    for b in (*d_taftps, d_ftpa, d_delays, d_w, d_cb, d_ib):
+    for b in (*d_taftps, d_ftpa, d_delays, *d_ws, d_cb, d_ib):
         b.free()
     if not args.no_extra_legs:
         result["pipeline_h2d_inclusive"] = pipeline_leg()

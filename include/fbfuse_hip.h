@@ -86,6 +86,10 @@ int fbf_device_synchronize(void);
 int fbf_event_create(fbf_event_t* event);
 int fbf_event_destroy(fbf_event_t event);
 int fbf_event_record(fbf_event_t event, fbf_stream_t stream);
+/* Work enqueued on `stream` after this call waits for `event` (its last
+ * record) to complete: the cross-stream ordering of a pipeline that runs
+ * the weights generator beside the split-transpose. */
+int fbf_stream_wait_event(fbf_stream_t stream, fbf_event_t event);
 int fbf_event_synchronize(fbf_event_t event);
 int fbf_event_elapsed_ms(fbf_event_t start, fbf_event_t stop, float* ms);
 

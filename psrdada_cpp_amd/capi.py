@@ -90,6 +90,7 @@ _SIGS = {
     "fbf_event_create": [pvp],
     "fbf_event_destroy": [vp],
     "fbf_event_record": [vp, vp],
+    "fbf_stream_wait_event": [vp, vp],
     "fbf_event_synchronize": [vp],
     "fbf_event_elapsed_ms": [vp, vp, C.POINTER(C.c_float)],
     "fbf_synth_gaussian_int8": [C.c_uint64, C.c_double, i32, vp, sz],
@@ -213,6 +214,10 @@ class Stream:
 
     def sync(self):
         call("fbf_stream_synchronize", self.handle)
+
+    def wait(self, event):
+        """Later work on this stream waits for `event` (fbf_stream_wait_event)."""
+        call("fbf_stream_wait_event", self.handle, event.handle)
 
     def close(self):
         if self.handle:

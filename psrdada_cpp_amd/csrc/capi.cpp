@@ -179,6 +179,13 @@ extern "C" int fbf_event_record(fbf_event_t event, fbf_stream_t stream)
     return FBF_OK;
 }
 
+extern "C" int fbf_stream_wait_event(fbf_stream_t stream, fbf_event_t event)
+{
+    FBF_REQUIRE(event, FBF_ERR_ARG, "fbf_stream_wait_event: null event");
+    FBF_HIP_TRY(hipStreamWaitEvent(static_cast<hipStream_t>(stream), static_cast<hipEvent_t>(event), 0));
+    return FBF_OK;
+}
+
 extern "C" int fbf_event_synchronize(fbf_event_t event)
 {
     FBF_REQUIRE(event, FBF_ERR_ARG, "fbf_event_synchronize: null event");

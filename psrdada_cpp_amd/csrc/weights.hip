@@ -13,39 +13,46 @@
 // (phase - rint(phase), exact), the reduced argument in [-1/2, 1/2] goes to
 // FP32 sincospif (HIP's device math library), then rounds to the nearest
 // integer, ties to even, as __float2int_rn does in WeightsManager.cu:70-71.
-// One thread per weight, antennas fastest: reads of the delays and writes of
-// the weights are coalesced.
+// A thread takes one (beam, antenna) over a run of kWChans channels: the
+// delay model is read and tau formed once, and each channel costs one FP64
+// product, the reduction, sincospif and the rounding.  Antennas are fastest
+// across lanes, so the delay reads and each channel's weight writes are
+// coalesced; the grid's y dimension runs over the channel runs.
 #include "fbf_internal.hpp"
 
 namespace fbf {
 namespace {
 
+constexpr int kWChans = 8;
+
 __global__ __launch_bounds__(256) void weights_k(const float2* __restrict__ delays,
-                                                 const double* __restrict__ freqs, double epoch, int A, int B,
-                                                 long long total, char2* __restrict__ weights,
+                                                 const double* __restrict__ freqs, double epoch, int F,
+                                                 int nd, char2* __restrict__ weights,
                                                  float* __restrict__ phase_out)
 {
 #pragma clang fp contract(off)
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    const long long ab = (long long)A * B;
-    const int f = (int)(idx / ab);
-    const long long di = idx - (long long)f * ab;  // b * A + a
+    const int di = blockIdx.x * blockDim.x + threadIdx.x;   // b * A + a
+    if (di >= nd) return;
     const float2 d = delays[di];
     double tau = (double)d.x * epoch;
     tau = tau + (double)d.y;
-    const double phase = tau * freqs[f];
-    const double frac = phase - rint(phase);
-    const float x = (float)frac;
-    float s, c;
-    sincospif(2.0f * x, &s, &c);
-    const float re = 127.0f * c;
-    const float im = 127.0f * s;
-    char2 w;
-    w.x = (signed char)(int)rintf(re);
-    w.y = (signed char)(int)rintf(im);
-    weights[idx] = w;
-    if (phase_out) phase_out[idx] = x;
+    const int f0 = blockIdx.y * kWChans;
+    const int f1 = f0 + kWChans < F ? f0 + kWChans : F;
+    for (int f = f0; f < f1; ++f) {
+        const double phase = tau * freqs[f];
+        const double frac = phase - rint(phase);
+        const float x = (float)frac;
+        float s, c;
+        sincospif(2.0f * x, &s, &c);
+        const float re = 127.0f * c;
+        const float im = 127.0f * s;
+        char2 w;
+        w.x = (signed char)(int)rintf(re);
+        w.y = (signed char)(int)rintf(im);
+        const size_t idx = (size_t)f * nd + di;
+        weights[idx] = w;
+        if (phase_out) phase_out[idx] = x;
+    }
 }
 
 }  // namespace
@@ -100,11 +107,13 @@ extern "C" int fbf_weights_generate(fbf_weights_plan* plan, const void* delays, 
                     std::to_string(total * 2));
     FBF_REQUIRE((reinterpret_cast<uintptr_t>(delays) & 7) == 0 && (reinterpret_cast<uintptr_t>(weights) & 1) == 0,
                 FBF_ERR_ARG, "fbf_weights_generate: misaligned buffer");
-    const long long nblocks = (total + 255) / 256;
-    FBF_REQUIRE(nblocks < (1ll << 31), FBF_ERR_UNSUPPORTED, "fbf_weights_generate: grid too large");
-    weights_k<<<dim3((unsigned)nblocks), 256, 0, static_cast<hipStream_t>(stream)>>>(
-        static_cast<const float2*>(delays), plan->d_freqs, epoch, plan->A, plan->B, total,
-        static_cast<char2*>(weights), phase);
+    FBF_REQUIRE(nd < (1ull << 31), FBF_ERR_UNSUPPORTED, "fbf_weights_generate: too many (beam, antenna) pairs");
+    const unsigned gx = (unsigned)((nd + 255) / 256);
+    const unsigned gy = (unsigned)((plan->F + kWChans - 1) / kWChans);
+    FBF_REQUIRE(gy < 65536u, FBF_ERR_UNSUPPORTED, "fbf_weights_generate: too many channels");
+    weights_k<<<dim3(gx, gy), 256, 0, static_cast<hipStream_t>(stream)>>>(
+        static_cast<const float2*>(delays), plan->d_freqs, epoch, plan->F, (int)nd, static_cast<char2*>(weights),
+        phase);
     FBF_LAUNCH_CHECK("weights_k");
     return FBF_OK;
 }

@@ -346,6 +346,22 @@ def test_weights_real_value(gpu, oracle):
     assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
 
 
+@pytest.mark.parametrize("A,B,F", [(5, 7, 13), (3, 100, 9), (64, 5, 1)])
+def test_weights_ragged_geometry(gpu, oracle, A, B, F):
+    """(beam, antenna) counts that leave a partial work-group and channel counts
+    that leave a partial run of channels per thread: every component written,
+    the reduced phase equal to the FP64 oracle's rounded to FP32, weights
+    within 1 LSB."""
+    capi = gpu
+    from psrdada_cpp_amd import workloads as wl
+    d = wl.synth_delays(B, A, 77 + A + B + F, 2e-6, rate_max=1e-9)
+    fr = capi.channel_frequencies_f64(wl.L_BAND_CENTRE_HZ, F * wl.CHANNEL_BW_HZ, F)
+    w, ph = _weights_gpu(capi, d, fr, 123.4, A, B, F)
+    ref, frac = oracle.weights_f64(d, fr, 123.4, A, B, F, want_phase=True)
+    np.testing.assert_array_equal(ph, frac.astype(np.float32))
+    assert np.abs(w.astype(int) - ref.astype(int)).max() <= 1
+
+
 @pytest.mark.parametrize("epoch", [0.0, 75.0, 300.0, 599.9])
 def test_weights_fp64_phase_config4(gpu, oracle, epoch):
     """64 antennas x 1024 beams x 64 channels, tau0 ~ U(-2us, 2us), rate ~ U(-1e-9, 1e-9),

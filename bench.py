@@ -626,11 +626,13 @@ def batch_leg(K, warmup):
             "x_real_time": round(cfg.block_seconds * 1e3 / wall_ms, 2),
             "int8_tops": round(ops / (wall_ms * 1e-3) / 1e12, 2),
             "autotune_s": round(t_tune, 2),
+            "schedule": "in turn on one stream" if batch.serial else "subarrays side by side",
         }
         batch.close()
         for b in (d_taftp, *ws, *outs):
             b.free()
-    out["workload"] = "64-ant block split 40/16/8 ant with 768/256/64 beams + IB, one hipGraph per block"
+    out["workload"] = ("64-ant block split 40/16/8 ant with 768/256/64 beams + IB, one hipGraph per block; "
+                       "the autotuner picks each subarray's kernel and the block's schedule")
     return out
 
 

@@ -121,6 +121,8 @@ _SIGS = {
     "fbf_batch_weights_bytes": [vp, i32, C.POINTER(C.c_size_t)],
     "fbf_batch_set_variant": [vp, i32, i32, i32],
     "fbf_batch_autotune": [vp, i32],
+    "fbf_batch_set_serial": [vp, i32],
+    "fbf_batch_get_serial": [vp, vp],
     "fbf_batch_run": [vp, vp, sz, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), i32, vp],
     "fbf_cb_beamform_ws": [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp],
     "fbf_cb_beamform_noalloc": [vp, vp, sz, vp, sz, vp, sz, vp],
@@ -543,6 +545,16 @@ class Batch:
 
     def autotune(self, reps=3):
         call("fbf_batch_autotune", self.handle, reps)
+
+    def set_serial(self, serial):
+        """1: every kernel of a block in turn on the caller's stream; 0: subarrays side by side."""
+        call("fbf_batch_set_serial", self.handle, int(bool(serial)))
+
+    @property
+    def serial(self):
+        v = C.c_int()
+        call("fbf_batch_get_serial", self.handle, C.byref(v))
+        return bool(v.value)
 
     def run(self, taftp, weights, outputs, use_graph=True, stream=None):
         w = (C.c_void_p * self.n)(*[x.ptr.value for x in weights])

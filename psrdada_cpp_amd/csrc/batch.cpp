@@ -30,6 +30,13 @@ struct fbf_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<const void*> graph_ptrs;   // pointers the graph was captured with
     hipStream_t graph_stream = nullptr;
+    // Every kernel of a block in turn on the caller's stream, instead of the
+    // subarrays side by side on their own streams.  On this GPU a
+    // cross-stream event wait costs 15-20 us before the waiting stream
+    // moves on (measured between the transpose and the beamformers, and
+    // between a block's join and the next block), while subarrays side by
+    // side take about as long as in turn; fbf_batch_autotune times both.
+    bool serial = false;
 };
 
 static size_t sub_out_bytes(const fbf_batch* b, const fbf_subarray_desc& d)
@@ -167,6 +174,24 @@ extern "C" int fbf_batch_set_variant(fbf_batch* b, int sub, int variant, int chu
     return st;
 }
 
+extern "C" int fbf_batch_set_serial(fbf_batch* b, int serial)
+{
+    FBF_REQUIRE(b, FBF_ERR_ARG, "fbf_batch_set_serial: null batch");
+    b->serial = serial != 0;
+    b->graph_ptrs.clear();   // forces a new capture
+    return FBF_OK;
+}
+
+extern "C" int fbf_batch_get_serial(const fbf_batch* b, int* serial)
+{
+    FBF_REQUIRE(b && serial, FBF_ERR_ARG, "fbf_batch_get_serial: null argument");
+    *serial = b->serial ? 1 : 0;
+    return FBF_OK;
+}
+
+// Tunes every subarray's kernel (fbf_cb_autotune), then times whole blocks
+// as graph launches with the subarrays side by side and in turn, on
+// synthetic data of the batch's geometry, and keeps the faster schedule.
 extern "C" int fbf_batch_autotune(fbf_batch* b, int reps)
 {
     FBF_REQUIRE(b && reps > 0, FBF_ERR_ARG, "fbf_batch_autotune: bad argument");
@@ -174,7 +199,106 @@ extern "C" int fbf_batch_autotune(fbf_batch* b, int reps)
         int st = fbf_cb_autotune(p, (long long)b->nhg * b->S, reps, nullptr, nullptr, nullptr);
         if (st) return st;
     }
-    b->graph_ptrs.clear();
+    const int n = (int)b->subs.size();
+    std::vector<void*> bufs;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        for (void* p : bufs) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+        b->graph_ptrs.clear();   // the next run captures its own pointers
+    };
+    auto alloc = [&](size_t bytes, int fill, void** p) {
+        hipError_t e = hipMalloc(p, bytes);
+        if (e == hipSuccess) {
+            bufs.push_back(*p);
+            e = hipMemsetAsync(*p, fill, bytes, s);
+        }
+        return e;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    void* taftp = nullptr;
+    if (e == hipSuccess) e = alloc(b->ftpa_bytes, 0x11, &taftp);
+    std::vector<const void*> w(n);
+    std::vector<void*> o(n + 1, nullptr);
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        size_t wb = 0, ob = 0;
+        fbf_batch_weights_bytes(b, i, &wb);
+        fbf_batch_output_bytes(b, i, &ob);
+        void* p = nullptr;
+        e = alloc(wb, 0x5A, &p);
+        w[i] = p;
+        if (e == hipSuccess) e = alloc(ob, 0, &o[i]);
+    }
+    if (e == hipSuccess && b->ib) {
+        size_t ob = 0;
+        fbf_batch_output_bytes(b, n, &ob);
+        e = alloc(ob, 0, &o[n]);
+    }
+    if (e != hipSuccess) {
+        cleanup();
+        return hip_status(e, "fbf_batch_autotune setup");
+    }
+    float best_ms = 0.0f;
+    bool best_serial = b->serial;
+    for (int mode = 0; mode < 2; ++mode) {
+        b->serial = mode == 1;
+        b->graph_ptrs.clear();
+        int st = FBF_OK;
+        for (int r = 0; r < 2 && st == FBF_OK; ++r) st = fbf_batch_run(b, taftp, b->ftpa_bytes, w.data(), o.data(), 1, s);
+        if (st == FBF_OK) e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && st == FBF_OK && e == hipSuccess; ++r)
+            st = fbf_batch_run(b, taftp, b->ftpa_bytes, w.data(), o.data(), 1, s);
+        if (st == FBF_OK && e == hipSuccess) e = hipEventRecord(e1, s);
+        if (st == FBF_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+        if (st != FBF_OK || e != hipSuccess) {
+            (void)hipStreamSynchronize(s);
+            cleanup();
+            return st != FBF_OK ? st : hip_status(e, "fbf_batch_autotune timing");
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (mode == 0 || ms < best_ms) {
+            best_ms = ms;
+            best_serial = b->serial;
+        }
+    }
+    b->serial = best_serial;
+    cleanup();
+    return FBF_OK;
+}
+
+// Enqueues one block in turn on `stream`: incoherent beam (when not fused),
+// transpose (with the fused incoherent beam), then each subarray.
+static int batch_enqueue_serial(fbf_batch* b, const void* taftp, size_t taftp_bytes, const void* const* weights,
+                                void* const* outputs, hipStream_t stream)
+{
+    const int n = (int)b->subs.size();
+    int st = FBF_OK;
+    if (b->ib) {
+        size_t ob = 0;
+        st = fbf_ib_output_bytes(b->ib, taftp_bytes, &ob);
+        if (st) return st;
+        st = b->ib_fused ? fbf_split_transpose_ib(taftp, taftp_bytes, b->ftpa, b->ftpa_bytes, b->F, b->S, b->ib,
+                                                  outputs[n], ob, stream)
+                         : fbf_ib_beamform(b->ib, taftp, taftp_bytes, outputs[n], ob, stream);
+        if (st) return st;
+    }
+    if (!b->ib || !b->ib_fused) {
+        st = fbf_split_transpose(taftp, taftp_bytes, b->ftpa, b->ftpa_bytes, b->A, b->A, 0, b->F, b->S, stream);
+        if (st) return st;
+    }
+    for (int i = 0; i < n; ++i) {
+        size_t wb = 0;
+        fbf_batch_weights_bytes(b, i, &wb);
+        st = fbf_cb_beamform_subarray(b->plans[i], b->ftpa, b->ftpa_bytes, b->A, b->subs[i].start_antenna, weights[i],
+                                      wb, outputs[i], sub_out_bytes(b, b->subs[i]), stream);
+        if (st) return st;
+    }
     return FBF_OK;
 }
 
@@ -182,6 +306,7 @@ extern "C" int fbf_batch_autotune(fbf_batch* b, int reps)
 static int batch_enqueue(fbf_batch* b, const void* taftp, size_t taftp_bytes, const void* const* weights,
                          void* const* outputs, hipStream_t stream)
 {
+    if (b->serial) return batch_enqueue_serial(b, taftp, taftp_bytes, weights, outputs, stream);
     const int n = (int)b->subs.size();
     // The incoherent beam comes out of the split-transpose pass when its
     // scrunch allows (fbf_split_transpose_ib), else it runs on its own

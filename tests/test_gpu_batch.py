@@ -47,18 +47,25 @@ def test_cb_subarray_view_equals_split_then_beamform(gpu, oracle, ts):
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_40_16_8_matches_oracle(gpu, oracle, use_graph):
-    """Every subarray's output (and the incoherent beam) equals the oracle.  With
-    the graph, it is replayed several times with eager work and host syncs in
-    between, and every replay is checked."""
+@pytest.mark.parametrize("serial", [False, True])
+@pytest.mark.parametrize("ib_fs", [1, 2])
+def test_batch_40_16_8_matches_oracle(gpu, oracle, use_graph, serial, ib_fs):
+    """Every subarray's output (and the incoherent beam) equals the oracle, with
+    the subarrays side by side or in turn on one stream, and with the
+    incoherent beam fused into the transpose (fscrunch 1) or on its own
+    (fscrunch 2).  With the graph, it is replayed several times with eager
+    work and host syncs in between, and every replay is checked."""
     capi = gpu
     F, nhg, ts = 8, 2, 16
     T = nhg * 256
     subs = [dict(sd, tscrunch=ts, fscrunch=1, nsamples_per_heap_out=T // ts) for sd in SPLIT]
     levels = [_levels(oracle, sd["nantennas"], ts, 1, F) for sd in subs]
     batch = capi.Batch(A_TOT, F, nhg, subs, levels)
-    _, _, ibo, ibs = oracle.power_levels(16.0, 24.0, 8, 16, 1, A_TOT, 16, 1)
-    batch.enable_ib(16, 1, np.full(F, ibo, np.float32), np.full(F, ibs, np.float32))
+    batch.set_serial(serial)
+    assert batch.serial == serial
+    _, _, ibo, ibs = oracle.power_levels(16.0, 24.0, 8, 16, 1, A_TOT, 16, ib_fs)
+    Fi = F // ib_fs
+    batch.enable_ib(16, ib_fs, np.full(Fi, ibo, np.float32), np.full(Fi, ibs, np.float32))
     ws = [unit_weights(3200 + k, F * sd["nbeams"] * sd["nantennas"]) for k, sd in enumerate(subs)]
     d_ws = [capi.DeviceBuffer.from_array(w) for w in ws]
     outs = [capi.DeviceBuffer(batch.output_bytes(i)) for i in range(len(subs) + 1)]
@@ -78,9 +85,9 @@ def test_batch_40_16_8_matches_oracle(gpu, oracle, use_graph):
         for k, sd in enumerate(subs):
             ref = _reference(oracle, taftp, nhg, F, sd, ts, 1, levels[k][0], levels[k][1], T // ts, ws[k])
             np.testing.assert_array_equal(outs[k].download(), ref, err_msg=f"replay {rep} subarray {k}")
-        pib = oracle.ib_power(taftp, F, 16, 1, nhg, A_TOT)
+        pib = oracle.ib_power(taftp, F, 16, ib_fs, nhg, A_TOT)
         np.testing.assert_array_equal(outs[-1].download(),
-                                      oracle.ib_tf(pib, np.full(F, ibo, np.float32), np.full(F, ibs, np.float32)))
+                                      oracle.ib_tf(pib, np.full(Fi, ibo, np.float32), np.full(Fi, ibs, np.float32)))
 
 
 def test_autotune_picks_a_variant_that_is_bit_exact(gpu, oracle):

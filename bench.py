@@ -111,14 +111,7 @@ def main():
                 for i in range(NBLK)]
     d_ftpa = capi.DeviceBuffer(cfg.ftpa_bytes(sub))
     d_delays = capi.DeviceBuffer.from_array(delays, s)
-    # Two weights buffers: block k's weights are generated on a second stream
-    # while the split-transpose of block k runs, into the buffer the CB of
-    # block k - 2 read (its completion event orders the reuse).
-    s_w = capi.Stream()
-    d_ws = [capi.DeviceBuffer(cfg.weights_bytes(sub)) for _ in range(2)]
-    w_ready = [capi.Event() for _ in range(2)]
-    cb_done = [capi.Event() for _ in range(2)]
-    wstate = {"gen": 0, "cur": 0, "cb_recorded": [False, False]}
+    d_w = capi.DeviceBuffer(cfg.weights_bytes(sub))
     wp = capi.WeightsPlan(A, B, F, freqs)
     cb = capi.CbPlan(A, B, F, sub.tscrunch, sub.fscrunch, cb_off, cb_sc,
                      nsamples_per_heap_out=cfg.cb_ohs(sub))
@@ -134,37 +127,33 @@ def main():
              and not args.separate_ib)
 
     def step(k, ev=None, weights_every=1):
-        """One block.  With `ev` (five events) every kernel runs on stream s
-        in turn, for per-kernel durations; otherwise the weights go on s_w
-        beside the split-transpose."""
+        """One block on stream s.  A block that takes new weights generates
+        them in the split-transpose's launch (fbf_split_transpose_ib_weights);
+        with `ev` (five events) every kernel runs on its own, in turn, for
+        per-kernel durations."""
         d_taftp = d_taftps[k % NBLK]
         new_w = k % weights_every == 0
-        if new_w:
-            b = wstate["gen"] % 2
-            wstate["gen"] += 1
-            wstate["cur"] = b
-        d_w = d_ws[wstate["cur"]]
+        epoch = cfg.block_seconds * k
         if ev:
             ev[0].record(s)
             if new_w:
-                wp.generate(d_delays, cfg.block_seconds * k, d_w, stream=s)
+                wp.generate(d_delays, epoch, d_w, stream=s)
             ev[1].record(s)
-        elif new_w:
-            if wstate["cb_recorded"][b]:
-                s_w.wait(cb_done[b])
-            wp.generate(d_delays, cfg.block_seconds * k, d_w, stream=s_w)
-            w_ready[b].record(s_w)
-        if fused:
-            capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
-        else:
-            capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
-        if ev:
+            if fused:
+                capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
+            else:
+                capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
             ev[2].record(s)
-        elif new_w:
-            s.wait(w_ready[b])
+        elif fused and new_w:
+            capi.split_transpose_ib_weights(d_taftp, d_ftpa, F, ib, d_ib, wp, d_delays, epoch, d_w, stream=s)
+        else:
+            if new_w:
+                wp.generate(d_delays, epoch, d_w, stream=s)
+            if fused:
+                capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
+            else:
+                capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
         cb.beamform(d_ftpa, d_w, d_cb, stream=s)
-        cb_done[wstate["cur"]].record(s)
-        wstate["cb_recorded"][wstate["cur"]] = True
         if ev:
             ev[3].record(s)
         if not fused:
@@ -217,10 +206,23 @@ def main():
     for r in range(R):
         step(args.warmup + 2 * K + r, kev[r])
     s.sync()
-    w_ms = np.mean([kev[r][0].elapsed_ms(kev[r][1]) for r in range(R)])
-    st_ms = np.mean([kev[r][1].elapsed_ms(kev[r][2]) for r in range(R)])
-    cb_ms = np.mean([kev[r][2].elapsed_ms(kev[r][3]) for r in range(R)])
-    ib_ms = np.mean([kev[r][3].elapsed_ms(kev[r][4]) for r in range(R)])
+    # Medians: the pass starts from an idle queue, so the first blocks can
+    # include host-side launch gaps.
+    w_ms = np.median([kev[r][0].elapsed_ms(kev[r][1]) for r in range(R)])
+    st_ms = np.median([kev[r][1].elapsed_ms(kev[r][2]) for r in range(R)])
+    cb_ms = np.median([kev[r][2].elapsed_ms(kev[r][3]) for r in range(R)])
+    ib_ms = np.median([kev[r][3].elapsed_ms(kev[r][4]) for r in range(R)])
+    # The launch the timed blocks use: split-transpose + IB + weights.
+    stw_ms = None
+    if fused:
+        fe = [capi.Event() for _ in range(R + 1)]
+        fe[0].record(s)
+        for r in range(R):
+            capi.split_transpose_ib_weights(d_taftps[r % NBLK], d_ftpa, F, ib, d_ib, wp, d_delays,
+                                            cfg.block_seconds * r, d_w, stream=s)
+            fe[r + 1].record(s)
+        s.sync()
+        stw_ms = np.median([fe[r].elapsed_ms(fe[r + 1]) for r in range(R)])
 
     ops = cfg.cb_ops()
     block_s = cfg.block_seconds
@@ -233,6 +235,8 @@ def main():
     st_name = "split_transpose_with_incoherent_beam" if fused else "split_transpose"
     kernel_ms = {"weights": round(float(w_ms), 4), st_name: round(float(st_ms), 4),
                  "coherent_beamformer": round(float(cb_ms), 4)}
+    if stw_ms is not None:
+        kernel_ms["split_transpose_ib_weights_one_launch"] = round(float(stw_ms), 4)
     kernel_gbs = {st_name: round(st_bytes / (st_ms * 1e-3) / 1e9, 1)}
     if not fused:
         kernel_ms["incoherent_beamformer"] = round(float(ib_ms), 4)
@@ -260,10 +264,8 @@ def main():
             "cb_tscrunch": sub.tscrunch, "cb_fscrunch": sub.fscrunch,
             "ib_tscrunch": cfg.ib_tscrunch, "ib_fscrunch": cfg.ib_fscrunch,
             "block_ms": round(block_s * 1e3, 3),
-            "stages": ("weights (every block, on a second stream beside the split-transpose) + split-transpose "
-                       "with IB in the same pass + CB" if fused
-                       else "weights (every block, on a second stream beside the split-transpose) + "
-                            "split-transpose + CB + IB"),
+            "stages": ("weights (every block), split-transpose and IB in one launch, then CB" if fused
+                       else "weights (every block) + split-transpose + CB + IB"),
             "input_blocks_rotating": NBLK,
             "device": name.value.decode(),
         },
@@ -297,12 +299,14 @@ def main():
         },
     }
 
-    gpu_weights = d_ws[wstate["cur"]].download()   # the last block's weights, for the CPU baseline's check
-    gpu_weights_epoch = cfg.block_seconds * (args.warmup + 2 * K + R - 1)
+    gpu_weights = d_w.download()      # the last block's weights, for the CPU baseline's check
+    # the epoch of the last weights written to d_w (the fused launches above
+    # when the block uses them, else the per-kernel pass)
+    gpu_weights_epoch = cfg.block_seconds * ((R - 1) if fused else (args.warmup + 2 * K + R - 1))
     wp.close()
     cb.close()
     ib.close()
-    for b in (*d_taftps, d_ftpa, d_delays, *d_ws, d_cb, d_ib):
+    for b in (*d_taftps, d_ftpa, d_delays, d_w, d_cb, d_ib):
         b.free()
     if not args.no_extra_legs:
         result["pipeline_h2d_inclusive"] = pipeline_leg()

@@ -271,6 +271,18 @@ int fbf_weights_plan_destroy(fbf_weights_plan* plan);
 int fbf_weights_generate(fbf_weights_plan* plan, const void* delays, size_t delays_bytes,
                          double epoch, void* weights, size_t weights_bytes,
                          float* phase, fbf_stream_t stream);
+/* fbf_split_transpose_ib plus fbf_weights_generate(weights_plan, delays,
+ * delays_bytes, epoch, weights, weights_bytes, NULL, stream) in one launch:
+ * the launch's first work-groups generate the weights while the rest do the
+ * transpose.  Outputs equal those of the two calls.  No counterpart in the
+ * reference, whose Pipeline (Pipeline.cu:164-166) generates the weights and
+ * then transposes in separate launches.  The weights plan's channels must
+ * be the block's. */
+int fbf_split_transpose_ib_weights(const void* taftp, size_t taftp_bytes, void* ftpa, size_t ftpa_bytes,
+                                   int nchans, int nsamples_per_heap, fbf_ib_plan* ib_plan, void* tf,
+                                   size_t tf_bytes, fbf_weights_plan* weights_plan, const void* delays,
+                                   size_t delays_bytes, double epoch, void* weights, size_t weights_bytes,
+                                   fbf_stream_t stream);
 
 /* ---- batched subarrays on one block (no counterpart in the reference:
  * it compiles one subarray per process, fbfuse_constants.hpp:12-14) ---- */

@@ -127,6 +127,7 @@ _SIGS = {
     "fbf_cb_beamform_ws": [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp],
     "fbf_cb_beamform_noalloc": [vp, vp, sz, vp, sz, vp, sz, vp],
     "fbf_split_transpose_ib": [vp, sz, vp, sz, i32, i32, vp, vp, sz, vp],
+    "fbf_split_transpose_ib_weights": [vp, sz, vp, sz, i32, i32, vp, vp, sz, vp, vp, sz, C.c_double, vp, sz, vp],
     "fbf_tuse_transpose": [vp, sz, vp, sz, i32, i32, i32, i32, i32, i32, i32, vp],
     "fbf_feng_to_dada": [vp, vp, sz, i32, vp],
     "fbf_cb_plan_reserve": [vp, C.c_longlong],
@@ -362,6 +363,14 @@ def split_transpose_ib(taftp, ftpa, nchans, ib_plan, tf, nsamples_per_heap=256, 
     """Full-antenna split-transpose and incoherent beam in one pass."""
     call("fbf_split_transpose_ib", taftp.ptr, taftp.nbytes, ftpa.ptr, ftpa.nbytes, nchans, nsamples_per_heap,
          ib_plan.handle, tf.ptr, tf.nbytes, _s(stream))
+
+
+def split_transpose_ib_weights(taftp, ftpa, nchans, ib_plan, tf, weights_plan, delays, epoch, weights,
+                               nsamples_per_heap=256, stream=None):
+    """split_transpose_ib and the weights generator in one launch."""
+    call("fbf_split_transpose_ib_weights", taftp.ptr, taftp.nbytes, ftpa.ptr, ftpa.nbytes, nchans,
+         nsamples_per_heap, ib_plan.handle, tf.ptr, tf.nbytes, weights_plan.handle, delays.ptr, delays.nbytes,
+         float(epoch), weights.ptr, weights.nbytes, _s(stream))
 
 
 def tuse_transpose(block, out, nchans, nsamples, nfreq, nbeams, ngroups, beam0=0, nbeams_out=None, stream=None):

@@ -49,6 +49,50 @@ struct IbFuseParams {
 };
 int ib_fuse_params(const fbf_ib_plan* plan, IbFuseParams* out);
 
+// Weights generation (weights.hip), shared by weights_k and by launches that
+// generate the weights beside other work.  A thread takes one (beam,
+// antenna) di over the run of kWChans channels starting at f0.
+constexpr int kWChans = 8;
+
+struct WeightsLaunch {
+    const float2* delays;   // [B][A] (rate, offset)
+    const double* freqs;    // device, [F]
+    double epoch;
+    int F, nd;              // channels; beams x antennas
+    char2* weights;         // [F][B][A]
+    unsigned gx, gy;        // work-groups over (beam, antenna) pairs and channel runs
+};
+// Checks a generate call's arguments against its plan and fills the launch.
+int weights_launch_params(const fbf_weights_plan* plan, const void* delays, size_t delays_bytes, double epoch,
+                          void* weights, size_t weights_bytes, WeightsLaunch* out);
+
+// phase = (rate * t + offset) * f in FP64 (contraction off, the oracle's
+// order), the whole-cycle part removed, then FP32 sincospif of the reduced
+// phase and both components rounded to the nearest integer, ties to even.
+__device__ __forceinline__ void weights_run(const WeightsLaunch& w, float* phase_out, int di, int f0)
+{
+#pragma clang fp contract(off)
+    const float2 d = w.delays[di];
+    double tau = (double)d.x * w.epoch;
+    tau = tau + (double)d.y;
+    const int f1 = f0 + kWChans < w.F ? f0 + kWChans : w.F;
+    for (int f = f0; f < f1; ++f) {
+        const double phase = tau * w.freqs[f];
+        const double frac = phase - rint(phase);
+        const float x = (float)frac;
+        float s, c;
+        sincospif(2.0f * x, &s, &c);
+        const float re = 127.0f * c;
+        const float im = 127.0f * s;
+        char2 v;
+        v.x = (signed char)(int)rintf(re);
+        v.y = (signed char)(int)rintf(im);
+        const size_t idx = (size_t)f * w.nd + di;
+        w.weights[idx] = v;
+        if (phase_out) phase_out[idx] = x;
+    }
+}
+
 // Number of XCDs on MI355X; work-groups are dispatched round robin over them.
 constexpr int kNumXcd = 8;
 

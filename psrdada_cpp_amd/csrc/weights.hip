@@ -23,36 +23,11 @@
 namespace fbf {
 namespace {
 
-constexpr int kWChans = 8;
-
-__global__ __launch_bounds__(256) void weights_k(const float2* __restrict__ delays,
-                                                 const double* __restrict__ freqs, double epoch, int F,
-                                                 int nd, char2* __restrict__ weights,
-                                                 float* __restrict__ phase_out)
+__global__ __launch_bounds__(256) void weights_k(WeightsLaunch w, float* __restrict__ phase_out)
 {
-#pragma clang fp contract(off)
     const int di = blockIdx.x * blockDim.x + threadIdx.x;   // b * A + a
-    if (di >= nd) return;
-    const float2 d = delays[di];
-    double tau = (double)d.x * epoch;
-    tau = tau + (double)d.y;
-    const int f0 = blockIdx.y * kWChans;
-    const int f1 = f0 + kWChans < F ? f0 + kWChans : F;
-    for (int f = f0; f < f1; ++f) {
-        const double phase = tau * freqs[f];
-        const double frac = phase - rint(phase);
-        const float x = (float)frac;
-        float s, c;
-        sincospif(2.0f * x, &s, &c);
-        const float re = 127.0f * c;
-        const float im = 127.0f * s;
-        char2 w;
-        w.x = (signed char)(int)rintf(re);
-        w.y = (signed char)(int)rintf(im);
-        const size_t idx = (size_t)f * nd + di;
-        weights[idx] = w;
-        if (phase_out) phase_out[idx] = x;
-    }
+    if (di >= w.nd) return;
+    weights_run(w, phase_out, di, blockIdx.y * kWChans);
 }
 
 }  // namespace
@@ -93,8 +68,8 @@ extern "C" int fbf_weights_plan_destroy(fbf_weights_plan* plan)
     return FBF_OK;
 }
 
-extern "C" int fbf_weights_generate(fbf_weights_plan* plan, const void* delays, size_t delays_bytes, double epoch,
-                                    void* weights, size_t weights_bytes, float* phase, fbf_stream_t stream)
+int fbf::weights_launch_params(const fbf_weights_plan* plan, const void* delays, size_t delays_bytes, double epoch,
+                               void* weights, size_t weights_bytes, WeightsLaunch* out)
 {
     FBF_REQUIRE(plan && delays && weights, FBF_ERR_ARG, "fbf_weights_generate: null argument");
     const size_t nd = (size_t)plan->A * plan->B;
@@ -108,12 +83,20 @@ extern "C" int fbf_weights_generate(fbf_weights_plan* plan, const void* delays, 
     FBF_REQUIRE((reinterpret_cast<uintptr_t>(delays) & 7) == 0 && (reinterpret_cast<uintptr_t>(weights) & 1) == 0,
                 FBF_ERR_ARG, "fbf_weights_generate: misaligned buffer");
     FBF_REQUIRE(nd < (1ull << 31), FBF_ERR_UNSUPPORTED, "fbf_weights_generate: too many (beam, antenna) pairs");
-    const unsigned gx = (unsigned)((nd + 255) / 256);
     const unsigned gy = (unsigned)((plan->F + kWChans - 1) / kWChans);
     FBF_REQUIRE(gy < 65536u, FBF_ERR_UNSUPPORTED, "fbf_weights_generate: too many channels");
-    weights_k<<<dim3(gx, gy), 256, 0, static_cast<hipStream_t>(stream)>>>(
-        static_cast<const float2*>(delays), plan->d_freqs, epoch, plan->F, (int)nd, static_cast<char2*>(weights),
-        phase);
+    *out = WeightsLaunch{static_cast<const float2*>(delays), plan->d_freqs, epoch, plan->F, (int)nd,
+                         static_cast<char2*>(weights), (unsigned)((nd + 255) / 256), gy};
+    return FBF_OK;
+}
+
+extern "C" int fbf_weights_generate(fbf_weights_plan* plan, const void* delays, size_t delays_bytes, double epoch,
+                                    void* weights, size_t weights_bytes, float* phase, fbf_stream_t stream)
+{
+    WeightsLaunch w{};
+    const int st = weights_launch_params(plan, delays, delays_bytes, epoch, weights, weights_bytes, &w);
+    if (st) return st;
+    weights_k<<<dim3(w.gx, w.gy), 256, 0, static_cast<hipStream_t>(stream)>>>(w, phase);
     FBF_LAUNCH_CHECK("weights_k");
     return FBF_OK;
 }

@@ -286,6 +286,44 @@ def test_split_transpose_ib_fused(gpu, oracle, A, F, nhg, ts):
         np.testing.assert_array_equal(d_tf.download(), oracle.ib_tf(p_ref, off, sc, mode))
 
 
+@pytest.mark.parametrize("A,F,nhg,ts,B", [(64, 64, 4, 16, 512), (40, 8, 2, 32, 96), (8, 16, 3, 1, 7),
+                                           (24, 4, 1, 4, 300)])
+def test_split_transpose_ib_weights_fused(gpu, oracle, A, F, nhg, ts, B):
+    """The weights generated by the first work-groups of the transpose launch
+    equal fbf_weights_generate's byte for byte, and the FTPA and TF outputs
+    still equal the oracle's (beam counts that leave partial work-groups, and
+    channel counts below and above one channel run)."""
+    capi = gpu
+    from psrdada_cpp_amd import workloads as wl
+    taftp = gaussian_int8(1250 + A + ts, nhg * A * F * 256 * 4, 16.0)
+    _, _, ibo, ibs = oracle.power_levels(16.0, 24.0, A, 16, 1, A, ts, 1)
+    off, sc = np.full(F, ibo, np.float32), np.full(F, ibs, np.float32)
+    d_in = _dev(capi, taftp)
+    d_ftpa = capi.DeviceBuffer(taftp.nbytes)
+    plan = capi.IbPlan(A, F, ts, 1, off, sc)
+    d_tf = capi.DeviceBuffer(plan.output_bytes(taftp.nbytes))
+    fr = capi.channel_frequencies_f64(wl.L_BAND_CENTRE_HZ, F * wl.CHANNEL_BW_HZ, F)
+    wp = capi.WeightsPlan(A, B, F, fr)
+    d_d = _dev(capi, wl.synth_delays(B, A, 1260 + B, 2e-6, rate_max=1e-9))
+    d_w_sep = capi.DeviceBuffer(F * B * A * 2)
+    d_w = capi.DeviceBuffer(F * B * A * 2)
+    for epoch in (0.0, 321.5):
+        wp.generate(d_d, epoch, d_w_sep)
+        d_ftpa.fill(0x5A)
+        d_w.fill(0x33)
+        capi.split_transpose_ib_weights(d_in, d_ftpa, F, plan, d_tf, wp, d_d, epoch, d_w)
+        np.testing.assert_array_equal(d_w.download(), d_w_sep.download(), err_msg=f"epoch {epoch}")
+        np.testing.assert_array_equal(d_ftpa.download(), oracle.split_transpose(taftp, nhg, A, A, 0, F))
+        np.testing.assert_array_equal(d_tf.download(), oracle.ib_tf(oracle.ib_power(taftp, F, ts, 1, nhg, A),
+                                                                     off, sc))
+    # a weights plan over other channels is refused
+    wp2 = capi.WeightsPlan(A, B, 2 * F, np.concatenate([fr, fr]))
+    with pytest.raises(capi.FbfError) as ei:
+        capi.split_transpose_ib_weights(d_in, d_ftpa, F, plan, d_tf, wp2, d_d, 0.0,
+                                        capi.DeviceBuffer(2 * F * B * A * 2))
+    assert ei.value.code == capi.FBF_ERR_ARG
+
+
 def test_split_transpose_ib_rejects_fscrunch(gpu):
     capi = gpu
     A, F, nhg = 8, 8, 1

@@ -271,6 +271,14 @@ int fbf_weights_plan_destroy(fbf_weights_plan* plan);
 int fbf_weights_generate(fbf_weights_plan* plan, const void* delays, size_t delays_bytes,
                          double epoch, void* weights, size_t weights_bytes,
                          float* phase, fbf_stream_t stream);
+/* Weights of ntsteps epochs tstart + k * tstep (k = 0 .. ntsteps - 1), written
+ * [t][F][B][A]: generate_weights_k's tstart/tstep/ntsteps
+ * (WeightsManager.cu:14-24, 58-73).  Step k equals fbf_weights_generate at
+ * epoch tstart + k * tstep (FP64 product, then sum).  weights_bytes is
+ * ntsteps * F * B * A * 2. */
+int fbf_weights_generate_steps(fbf_weights_plan* plan, const void* delays, size_t delays_bytes,
+                               double tstart, double tstep, int ntsteps, void* weights, size_t weights_bytes,
+                               fbf_stream_t stream);
 /* fbf_split_transpose_ib plus fbf_weights_generate(weights_plan, delays,
  * delays_bytes, epoch, weights, weights_bytes, NULL, stream) in one launch:
  * the launch's first work-groups generate the weights while the rest do the

@@ -152,6 +152,7 @@ _SIGS = {
     "fbf_weights_plan_create": [i32, i32, i32, vp, pvp],
     "fbf_weights_plan_destroy": [vp],
     "fbf_weights_generate": [vp, vp, sz, C.c_double, vp, sz, vp, vp],
+    "fbf_weights_generate_steps": [vp, vp, sz, C.c_double, C.c_double, i32, vp, sz, vp],
 }
 
 _lib = None
@@ -498,6 +499,11 @@ class WeightsPlan:
     def generate(self, delays, epoch, weights, phase=None, stream=None):
         call("fbf_weights_generate", self.handle, delays.ptr, delays.nbytes, float(epoch),
              weights.ptr, weights.nbytes, phase.ptr if phase is not None else None, _s(stream))
+
+    def generate_steps(self, delays, tstart, tstep, ntsteps, weights, stream=None):
+        """Weights of epochs tstart + k * tstep, k < ntsteps, as [t][F][B][A]."""
+        call("fbf_weights_generate_steps", self.handle, delays.ptr, delays.nbytes, float(tstart), float(tstep),
+             int(ntsteps), weights.ptr, weights.nbytes, _s(stream))
 
     def close(self):
         if self.handle:

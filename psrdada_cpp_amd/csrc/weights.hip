@@ -30,6 +30,22 @@ __global__ __launch_bounds__(256) void weights_k(WeightsLaunch w, float* __restr
     weights_run(w, phase_out, di, blockIdx.y * kWChans);
 }
 
+// Time steps of one delay model (WeightsManager.cu:14-79's tstart, tstep,
+// ntsteps): step k is blockIdx.z, at epoch tstart + k * tstep formed in
+// FP64 (product, then sum), written at weights + k * F * B * A.
+__global__ __launch_bounds__(256) void weights_steps_k(WeightsLaunch w, double tstart, double tstep)
+{
+#pragma clang fp contract(off)
+    const int di = blockIdx.x * blockDim.x + threadIdx.x;
+    if (di >= w.nd) return;
+    const int k = blockIdx.z;
+    WeightsLaunch wk = w;
+    const double dt = (double)k * tstep;
+    wk.epoch = tstart + dt;
+    wk.weights = w.weights + (size_t)k * w.F * w.nd;
+    weights_run(wk, nullptr, di, blockIdx.y * kWChans);
+}
+
 }  // namespace
 }  // namespace fbf
 
@@ -98,5 +114,22 @@ extern "C" int fbf_weights_generate(fbf_weights_plan* plan, const void* delays, 
     if (st) return st;
     weights_k<<<dim3(w.gx, w.gy), 256, 0, static_cast<hipStream_t>(stream)>>>(w, phase);
     FBF_LAUNCH_CHECK("weights_k");
+    return FBF_OK;
+}
+
+extern "C" int fbf_weights_generate_steps(fbf_weights_plan* plan, const void* delays, size_t delays_bytes,
+                                          double tstart, double tstep, int ntsteps, void* weights,
+                                          size_t weights_bytes, fbf_stream_t stream)
+{
+    FBF_REQUIRE(ntsteps > 0 && ntsteps <= 65535, FBF_ERR_ARG, "fbf_weights_generate_steps: ntsteps must be 1..65535");
+    FBF_REQUIRE(weights_bytes % (size_t)ntsteps == 0, FBF_ERR_SIZE,
+                "fbf_weights_generate_steps: weights size is not ntsteps whole steps");
+    WeightsLaunch w{};
+    const int st =
+        weights_launch_params(plan, delays, delays_bytes, tstart, weights, weights_bytes / (size_t)ntsteps, &w);
+    if (st) return st;
+    weights_steps_k<<<dim3(w.gx, w.gy, (unsigned)ntsteps), 256, 0, static_cast<hipStream_t>(stream)>>>(w, tstart,
+                                                                                                      tstep);
+    FBF_LAUNCH_CHECK("weights_steps_k");
     return FBF_OK;
 }

@@ -400,6 +400,33 @@ def test_weights_ragged_geometry(gpu, oracle, A, B, F):
     assert np.abs(w.astype(int) - ref.astype(int)).max() <= 1
 
 
+@pytest.mark.parametrize("A,B,F,ntsteps", [(64, 96, 16, 5), (5, 7, 13, 3), (8, 256, 64, 1)])
+def test_weights_time_steps(gpu, oracle, A, B, F, ntsteps):
+    """generate_weights_k's tstart/tstep/ntsteps (WeightsManager.cu:14-24,
+    58-73): step k of the [t][F][B][A] output equals the single-epoch
+    generator at tstart + k * tstep, and is within 1 LSB of the FP64 oracle."""
+    capi = gpu
+    from psrdada_cpp_amd import workloads as wl
+    d = wl.synth_delays(B, A, 2024 + A + B, 2e-6, rate_max=1e-9)
+    fr = capi.channel_frequencies_f64(wl.L_BAND_CENTRE_HZ, F * wl.CHANNEL_BW_HZ, F)
+    plan = capi.WeightsPlan(A, B, F, fr)
+    d_d = _dev(capi, d)
+    tstart, tstep = 12.25, 0.5360
+    d_all = capi.DeviceBuffer(ntsteps * F * B * A * 2)
+    d_all.fill(0x77)
+    plan.generate_steps(d_d, tstart, tstep, ntsteps, d_all)
+    allw = d_all.download().reshape(ntsteps, -1)
+    d_one = capi.DeviceBuffer(F * B * A * 2)
+    for k in range(ntsteps):
+        epoch = tstart + float(k) * tstep
+        plan.generate(d_d, epoch, d_one)
+        np.testing.assert_array_equal(allw[k], d_one.download(), err_msg=f"step {k}")
+        ref = oracle.weights_f64(d, fr, epoch, A, B, F)
+        assert np.abs(allw[k].astype(int) - ref.astype(int)).max() <= 1
+    with pytest.raises(capi.FbfError):
+        plan.generate_steps(d_d, tstart, tstep, ntsteps + 1, d_all)
+
+
 @pytest.mark.parametrize("epoch", [0.0, 75.0, 300.0, 599.9])
 def test_weights_fp64_phase_config4(gpu, oracle, epoch):
     """64 antennas x 1024 beams x 64 channels, tau0 ~ U(-2us, 2us), rate ~ U(-1e-9, 1e-9),

@@ -321,12 +321,14 @@ int fbf_batch_output_bytes(const fbf_batch* batch, int index, size_t* bytes);
 int fbf_batch_weights_bytes(const fbf_batch* batch, int sub, size_t* bytes);
 int fbf_batch_set_variant(fbf_batch* batch, int sub, int variant, int chunk);
 /* Tunes each subarray's kernel (fbf_cb_autotune), then times whole blocks
- * with the subarrays side by side on their own streams and in turn on the
- * caller's stream, and keeps the faster schedule. */
+ * with the subarrays in turn on the caller's stream (the default) and side
+ * by side on their own streams, and switches to side by side only if it is
+ * more than 3 % faster. */
 int fbf_batch_autotune(fbf_batch* batch, int reps);
-/* Schedule of a block: 0 = subarrays side by side (fork/join over the
- * batch's streams), 1 = every kernel in turn on the caller's stream.  No
- * reference counterpart (the reference has one subarray per pipeline). */
+/* Schedule of a block: 1 = every kernel in turn on the caller's stream (the
+ * default), 0 = subarrays side by side (fork/join over the batch's
+ * streams).  No reference counterpart (the reference has one subarray per
+ * pipeline). */
 int fbf_batch_set_serial(fbf_batch* batch, int serial);
 int fbf_batch_get_serial(const fbf_batch* batch, int* serial);
 /* Split-transposes the block once into a full-antenna FTPA workspace and

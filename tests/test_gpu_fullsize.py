@@ -3,9 +3,10 @@ against the oracle (CoherentBeamformerTester.cu:148-185 compares whole
 blocks the same way).
 
 * Config 2: one 64-antenna, 64-channel, 32-heap-group block (8192 spectra),
-  512 beams, tscrunch 16: weights from the delay generator, the fused
-  split-transpose + incoherent beam pass, then the coherent beamformer, with
-  the benchmark's seed.  The beamformer's one-round grid partition depends
+  512 beams, tscrunch 16, with the benchmark's seed and launches: the
+  split-transpose + incoherent beam launch that also generates the block's
+  weights from the delay models (fbf_split_transpose_ib_weights), then the
+  coherent beamformer.  The beamformer's one-round grid partition depends
   on F, T and the compute-unit count only, so this is the launch the
   benchmark times.
 * Config 3: the same block split 40/16/8 antennas with 768/256/64 beams plus
@@ -49,7 +50,11 @@ def test_config2_block_bit_exact(gpu, oracle, quant_mode):
     s = capi.Stream()
     d_taftp = capi.DeviceBuffer.from_array(taftp)
     d_ftpa = capi.DeviceBuffer(cfg.ftpa_bytes(sub))
-    d_w, w = _weights(capi, wl, cfg, sub, BENCH_SEED + 1)
+    delays = wl.synth_delays(B, A, BENCH_SEED + 1, cfg.tau_max_s)
+    d_delays = capi.DeviceBuffer.from_array(delays)
+    wp = capi.WeightsPlan(A, B, F, cfg.channel_frequencies())
+    d_w = capi.DeviceBuffer(cfg.weights_bytes(sub))
+    epoch = cfg.block_seconds * 7     # the 8th block of a run
     cb = capi.CbPlan(A, B, F, sub.tscrunch, sub.fscrunch, cb_off, cb_sc, nsamples_per_heap_out=cfg.cb_ohs(sub),
                      quant_mode=quant_mode)
     ib = capi.IbPlan(cfg.total_nantennas, F, cfg.ib_tscrunch, cfg.ib_fscrunch, ib_off, ib_sc, quant_mode=quant_mode)
@@ -57,12 +62,13 @@ def test_config2_block_bit_exact(gpu, oracle, quant_mode):
     d_ib = capi.DeviceBuffer(cfg.ib_out_bytes())
     d_ftpa.fill(0x5A, s)
     d_cb.fill(0x5A, s)
-    capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
+    d_w.fill(0x5A, s)
+    capi.split_transpose_ib_weights(d_taftp, d_ftpa, F, ib, d_ib, wp, d_delays, epoch, d_w, stream=s)
     cb.beamform(d_ftpa, d_w, d_cb, stream=s)
     s.sync()
+    w = d_w.download()
 
-    w_ref = oracle.weights_f64(wl.synth_delays(B, A, BENCH_SEED + 1, cfg.tau_max_s), cfg.channel_frequencies(),
-                               0.0, A, B, F)
+    w_ref = oracle.weights_f64(delays, cfg.channel_frequencies(), epoch, A, B, F)
     dw = np.abs(w.astype(int) - w_ref.astype(int))
     assert dw.max() <= 1 and (dw > 0).mean() < 1e-3
     ftpa = oracle.split_transpose(taftp, nhg, A, A, 0, F)

@@ -203,6 +203,7 @@ extern "C" int fbf_batch_autotune(fbf_batch* b, int reps)
         if (st) return st;
     }
     const int n = (int)b->subs.size();
+    const bool serial0 = b->serial;   // kept if the timing fails
     std::vector<void*> bufs;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -212,6 +213,7 @@ extern "C" int fbf_batch_autotune(fbf_batch* b, int reps)
         if (e1) (void)hipEventDestroy(e1);
         if (s) (void)hipStreamDestroy(s);
         b->graph_ptrs.clear();   // the next run captures its own pointers
+        b->graph_stream = nullptr;
     };
     auto alloc = [&](size_t bytes, int fill, void** p) {
         hipError_t e = hipMalloc(p, bytes);
@@ -260,6 +262,7 @@ extern "C" int fbf_batch_autotune(fbf_batch* b, int reps)
         if (st == FBF_OK && e == hipSuccess) e = hipEventSynchronize(e1);
         if (st != FBF_OK || e != hipSuccess) {
             (void)hipStreamSynchronize(s);
+            b->serial = serial0;
             cleanup();
             return st != FBF_OK ? st : hip_status(e, "fbf_batch_autotune timing");
         }

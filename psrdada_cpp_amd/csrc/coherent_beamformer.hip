@@ -2292,12 +2292,16 @@ static bool cb_ws_applies(const fbf_cb_desc& d, int mode, long long T)
            cb_ws_supported(d.nantennas, d.tscrunch, d.fscrunch);
 }
 
-// Automatic choice: the transposed kernel with 64 beams per wave, else with
-// 32 beams per wave, else the LDS-staged kernel (which also serves int32
-// sums, power output and the geometries the transposed kernel does not).
+// Automatic choice: up to three 32-byte K steps (48 antennas) the
+// wave-private kernel, which measured 20-85 % faster there (8 to 48
+// antennas, tscrunch 4 to 64; one exception, 32 antennas at tscrunch 4, 8 %
+// slower); else the transposed kernel with 64 beams per wave, else with 32
+// beams per wave, else the LDS-staged kernel (which also serves int32 sums,
+// power output and the geometries the transposed kernels do not).
 static CbVariant cb_pick_variant(const fbf_cb_desc& d, int variant, int mode, long long T)
 {
     if (variant <= 0) {
+        if ((2 * d.nantennas + 31) / 32 <= 3 && cb_ws_applies(d, mode, T)) return kCbVariants[kCbVariantWs];
         if (cb_tr_applies(d, 4, mode, T)) return kCbVariants[kCbVariantTr4];
         if (cb_tr_applies(d, 2, mode, T)) return kCbVariants[kCbVariantTr2];
         return kCbVariants[kCbVariantLds];

@@ -73,8 +73,8 @@ def percentile(x, q):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
-                                   "(the path shards by frequency channel with no exchange, see DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark measures one GPU; --gpus above 1 is not supported "
+                                   "(the path shards by frequency channel with no exchange between GPUs)"}))
         return 2
 
     if args.mfma_reference_only:

@@ -7,7 +7,7 @@
  * CPU fallback.
  *
  * Every function names the reference file:line whose behaviour it restates
- * (paths relative to the psrdada_cpp source tree).  Where this build fixes a
+ * (paths relative to the psrdada_cpp source tree).  Where the library fixes a
  * semantic the reference leaves open, the choice is written next to the
  * function and mirrored exactly by the HIP kernels:
  *

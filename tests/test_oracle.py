@@ -233,7 +233,7 @@ def test_cb_tbtf_layout(oracle):
     (4, 8192, 64, 512, 16, 1, 16.0, 24.0, True, 2e-5),
 ])
 def test_cb_reference_f32_agrees_within_reference_tolerance(oracle, F, T, A, B, ts, fs, lin, lout, clip, max_rate):
-    """The reference's FP32 host model vs this build's exact-power TRUNC path:
+    """The reference's FP32 host model vs the library's exact-power TRUNC path:
     within the reference's own +-1 (CoherentBeamformerTester.cu:144), and
     differing only where the FP32 accumulation of the power lands on the
     other side of a quantiser step: the rate is measured for these seeded

@@ -7,10 +7,14 @@ beams, tscrunch 16, plus the incoherent beam), with the voltages already
 resident in HBM (three distinct blocks, 403 MB together, rotate so that the
 input is not served from the 256 MB Infinity Cache):
     delay -> weights at the block's epoch (FP64 phase), every block as
-        Pipeline.cu:164 does
-    split-transpose TAFTP -> FTPA (with the incoherent beam in the same pass)
-    coherent beamformer (int8 MFMA, detect, scrunch, quantise) -> TBTF int8
-all on one stream.  Prints one JSON line.
+        Pipeline.cu:164 does, generated in the incoherent beam's launch
+    incoherent beam from TAFTP -> TF int8
+    coherent beamformer straight from TAFTP (staged and transposed in LDS,
+        no split-transpose pass; int8 MFMA, detect, scrunch, quantise)
+        -> TBTF int8
+all on one stream (--path ftpa: the reference's order instead, a
+split-transpose pass to FTPA with the incoherent beam in it, then the
+coherent beamformer on FTPA).  Prints one JSON line.
 
 value        = x real-time: block duration / wall time per block (K blocks
                timed back to back, device synchronised on both sides).
@@ -42,6 +46,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-channels", type=int, default=64, help="channels in the CPU sample (64 = whole block)")
     ap.add_argument("--variant", type=int, default=0, help="CB kernel variant (0 = auto)")
+    ap.add_argument("--path", choices=["taftp", "ftpa"], default="taftp",
+                    help="taftp: CB straight from TAFTP; ftpa: split-transpose pass, then CB on FTPA")
     ap.add_argument("--separate-ib", action="store_true",
                     help="run the incoherent beam as its own kernel instead of inside the split-transpose pass")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 40/16/8 subarray leg")
@@ -109,7 +115,8 @@ def main():
     s = capi.Stream()
     d_taftps = [capi.DeviceBuffer.from_array(taftp if i == 0 else wl.synth_taftp(cfg, seed + 100 + i), s)
                 for i in range(NBLK)]
-    d_ftpa = capi.DeviceBuffer(cfg.ftpa_bytes(sub))
+    taftp_path = args.path == "taftp"
+    d_ftpa = None if taftp_path else capi.DeviceBuffer(cfg.ftpa_bytes(sub))
     d_delays = capi.DeviceBuffer.from_array(delays, s)
     d_w = capi.DeviceBuffer(cfg.weights_bytes(sub))
     wp = capi.WeightsPlan(A, B, F, freqs)
@@ -121,14 +128,18 @@ def main():
     d_cb = capi.DeviceBuffer(cfg.cb_out_bytes(sub))
     d_ib = capi.DeviceBuffer(cfg.ib_out_bytes())
 
-    # The incoherent beam comes out of the split-transpose pass when the block
-    # is split whole and the IB scrunch allows it (fbf_split_transpose_ib).
-    fused = (A == cfg.total_nantennas and cfg.ib_fscrunch == 1 and 32 % cfg.ib_tscrunch == 0
+    # FTPA path: the incoherent beam comes out of the split-transpose pass
+    # when the block is split whole and the IB scrunch allows it
+    # (fbf_split_transpose_ib).
+    fused = (not taftp_path and A == cfg.total_nantennas and cfg.ib_fscrunch == 1 and 32 % cfg.ib_tscrunch == 0
              and not args.separate_ib)
+    if taftp_path and not cb.taftp_direct(cfg.total_nantennas, T):
+        raise RuntimeError("the coherent-beam plan does not read this block straight from TAFTP")
 
     def step(k, ev=None, weights_every=1):
         """One block on stream s.  A block that takes new weights generates
-        them in the split-transpose's launch (fbf_split_transpose_ib_weights);
+        them in the incoherent beam's launch (fbf_ib_beamform_weights; FTPA
+        path: in the split-transpose's, fbf_split_transpose_ib_weights);
         with `ev` (five events) every kernel runs on its own, in turn, for
         per-kernel durations."""
         d_taftp = d_taftps[k % NBLK]
@@ -139,11 +150,18 @@ def main():
             if new_w:
                 wp.generate(d_delays, epoch, d_w, stream=s)
             ev[1].record(s)
-            if fused:
+            if taftp_path:
+                ib.beamform(d_taftp, d_ib, stream=s)
+            elif fused:
                 capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
             else:
                 capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
             ev[2].record(s)
+        elif taftp_path:
+            if new_w:
+                ib.beamform_weights(d_taftp, d_ib, wp, d_delays, epoch, d_w, stream=s)
+            else:
+                ib.beamform(d_taftp, d_ib, stream=s)
         elif fused and new_w:
             capi.split_transpose_ib_weights(d_taftp, d_ftpa, F, ib, d_ib, wp, d_delays, epoch, d_w, stream=s)
         else:
@@ -153,10 +171,13 @@ def main():
                 capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
             else:
                 capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
-        cb.beamform(d_ftpa, d_w, d_cb, stream=s)
+        if taftp_path:
+            cb.beamform_taftp(d_taftp, d_w, d_cb, nantennas_total=cfg.total_nantennas, stream=s)
+        else:
+            cb.beamform(d_ftpa, d_w, d_cb, stream=s)
         if ev:
             ev[3].record(s)
-        if not fused:
+        if not fused and not taftp_path:
             ib.beamform(d_taftp, d_ib, stream=s)
         if ev:
             ev[4].record(s)
@@ -212,14 +233,18 @@ def main():
     st_ms = np.median([kev[r][1].elapsed_ms(kev[r][2]) for r in range(R)])
     cb_ms = np.median([kev[r][2].elapsed_ms(kev[r][3]) for r in range(R)])
     ib_ms = np.median([kev[r][3].elapsed_ms(kev[r][4]) for r in range(R)])
-    # The launch the timed blocks use: split-transpose + IB + weights.
+    # The launch the timed blocks use for a block's weights: IB + weights
+    # (FTPA path: split-transpose + IB + weights).
     stw_ms = None
-    if fused:
+    if fused or taftp_path:
         fe = [capi.Event() for _ in range(R + 1)]
         fe[0].record(s)
         for r in range(R):
-            capi.split_transpose_ib_weights(d_taftps[r % NBLK], d_ftpa, F, ib, d_ib, wp, d_delays,
-                                            cfg.block_seconds * r, d_w, stream=s)
+            if taftp_path:
+                ib.beamform_weights(d_taftps[r % NBLK], d_ib, wp, d_delays, cfg.block_seconds * r, d_w, stream=s)
+            else:
+                capi.split_transpose_ib_weights(d_taftps[r % NBLK], d_ftpa, F, ib, d_ib, wp, d_delays,
+                                                cfg.block_seconds * r, d_w, stream=s)
             fe[r + 1].record(s)
         s.sync()
         stw_ms = np.median([fe[r].elapsed_ms(fe[r + 1]) for r in range(R)])
@@ -230,15 +255,20 @@ def main():
     cb_tops = ops / (cb_ms * 1e-3) / 1e12
     pipeline_tops = ops / (ms_per_step * 1e-3) / 1e12
     samples_per_s = (F * T * wl.NPOL * cfg.total_nantennas) / (ms_per_step * 1e-3)
-    st_bytes = 2 * cfg.ftpa_bytes(sub) + (cfg.ib_out_bytes() if fused else 0)
     ib_bytes = cfg.taftp_bytes + cfg.ib_out_bytes()
-    st_name = "split_transpose_with_incoherent_beam" if fused else "split_transpose"
+    if taftp_path:
+        st_bytes = ib_bytes
+        st_name = "incoherent_beamformer"
+    else:
+        st_bytes = 2 * cfg.ftpa_bytes(sub) + (cfg.ib_out_bytes() if fused else 0)
+        st_name = "split_transpose_with_incoherent_beam" if fused else "split_transpose"
     kernel_ms = {"weights": round(float(w_ms), 4), st_name: round(float(st_ms), 4),
                  "coherent_beamformer": round(float(cb_ms), 4)}
     if stw_ms is not None:
-        kernel_ms["split_transpose_ib_weights_one_launch"] = round(float(stw_ms), 4)
+        kernel_ms["ib_weights_one_launch" if taftp_path else "split_transpose_ib_weights_one_launch"] = \
+            round(float(stw_ms), 4)
     kernel_gbs = {st_name: round(st_bytes / (st_ms * 1e-3) / 1e9, 1)}
-    if not fused:
+    if not fused and not taftp_path:
         kernel_ms["incoherent_beamformer"] = round(float(ib_ms), 4)
         kernel_gbs["incoherent_beamformer"] = round(ib_bytes / (ib_ms * 1e-3) / 1e9, 1)
 
@@ -264,7 +294,9 @@ def main():
             "cb_tscrunch": sub.tscrunch, "cb_fscrunch": sub.fscrunch,
             "ib_tscrunch": cfg.ib_tscrunch, "ib_fscrunch": cfg.ib_fscrunch,
             "block_ms": round(block_s * 1e3, 3),
-            "stages": ("weights (every block), split-transpose and IB in one launch, then CB" if fused
+            "stages": ("weights (every block) and the incoherent beam in one launch, then the coherent beams "
+                       "straight from TAFTP (no split-transpose pass)" if taftp_path
+                       else "weights (every block), split-transpose and IB in one launch, then CB" if fused
                        else "weights (every block) + split-transpose + CB + IB"),
             "input_blocks_rotating": NBLK,
             "device": name.value.decode(),
@@ -295,19 +327,22 @@ def main():
                               "on this workload, scaled by a plain copy's readings, from "
                               "profiles/cb_64ant_64ch_512beams_hbm_traffic.txt (not measured in this run)",
             "ops_per_launch": ops,
-            "window": "HIP events around fbf_cb_beamform on its stream: cb_tr_k plus the tbtf_k reorder",
+            "window": ("HIP events around fbf_cb_beamform_taftp on its stream: cb_tr_k (staging the TAFTP "
+                       "block itself) plus the tbtf_k reorder" if taftp_path else
+                       "HIP events around fbf_cb_beamform on its stream: cb_tr_k plus the tbtf_k reorder"),
         },
     }
 
     gpu_weights = d_w.download()      # the last block's weights, for the CPU baseline's check
     # the epoch of the last weights written to d_w (the fused launches above
     # when the block uses them, else the per-kernel pass)
-    gpu_weights_epoch = cfg.block_seconds * ((R - 1) if fused else (args.warmup + 2 * K + R - 1))
+    gpu_weights_epoch = cfg.block_seconds * ((R - 1) if (fused or taftp_path) else (args.warmup + 2 * K + R - 1))
     wp.close()
     cb.close()
     ib.close()
     for b in (*d_taftps, d_ftpa, d_delays, d_w, d_cb, d_ib):
-        b.free()
+        if b is not None:
+            b.free()
     if not args.no_extra_legs:
         result["pipeline_h2d_inclusive"] = pipeline_leg()
         result["weights_generation"] = weights_leg(args.warmup)

@@ -183,6 +183,25 @@ int fbf_cb_beamform_subarray(fbf_cb_plan* plan, const void* ftpa_block, size_t b
                              int block_nantennas, int antenna_offset,
                              const void* fba_weights, size_t weights_bytes,
                              void* tbtf, size_t tbtf_bytes, fbf_stream_t stream);
+/* Beamform straight from a TAFTP block: replaces the pair
+ * SplitTranspose::transpose + CoherentBeamformer::beamform as the
+ * reference's Pipeline calls them (Pipeline.cu:166-168), with the same
+ * output as fbf_split_transpose of antennas [start_antenna, start_antenna +
+ * plan nantennas) of the block's nantennas_total, then fbf_cb_beamform.
+ * Where fbf_cb_taftp_direct says so, the kernel stages the block's
+ * [antenna][sample, pol] runs itself and transposes them in LDS, so no FTPA
+ * copy is written; otherwise the call split-transposes into an FTPA buffer
+ * that the plan owns (grown on demand, as the staging buffer) first. */
+int fbf_cb_beamform_taftp(fbf_cb_plan* plan, const void* taftp, size_t taftp_bytes,
+                          int nantennas_total, int start_antenna, int nsamples_per_heap,
+                          const void* fba_weights, size_t weights_bytes,
+                          void* tbtf, size_t tbtf_bytes, fbf_stream_t stream);
+/* *direct = 1 when fbf_cb_beamform_taftp reads a block of this geometry
+ * without the FTPA copy: the plan's kernel is the transposed one with 64 or
+ * 128 beams per wave, fscrunch 1, nsamples_per_heap a power of two >= 16,
+ * and one heap group under 2 GiB.  No counterpart in the reference. */
+int fbf_cb_taftp_direct(const fbf_cb_plan* plan, int nantennas_total, int nsamples_per_heap,
+                        long long nsamples, int* direct);
 /* LDS chunk length (columns of the integration window staged per step),
  * 0 = automatic; a power of two <= 16 dividing 2*tscrunch. */
 int fbf_cb_plan_set_chunk(fbf_cb_plan* plan, int chunk);
@@ -291,6 +310,18 @@ int fbf_split_transpose_ib_weights(const void* taftp, size_t taftp_bytes, void* 
                                    size_t tf_bytes, fbf_weights_plan* weights_plan, const void* delays,
                                    size_t delays_bytes, double epoch, void* weights, size_t weights_bytes,
                                    fbf_stream_t stream);
+/* fbf_ib_beamform plus fbf_weights_generate(weights_plan, delays,
+ * delays_bytes, epoch, weights, weights_bytes, NULL, stream) in one launch,
+ * as fbf_split_transpose_ib_weights: the launch's first work-groups generate
+ * the weights while the rest form the incoherent beam.  Outputs equal those
+ * of the two calls.  It pairs with fbf_cb_beamform_taftp, which needs no
+ * split-transpose pass; no counterpart in the reference (Pipeline.cu:164-170
+ * launches the weights and the incoherent beam separately).  The weights
+ * plan's channels must be the block's. */
+int fbf_ib_beamform_weights(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes, void* tf,
+                            size_t tf_bytes, fbf_weights_plan* weights_plan, const void* delays,
+                            size_t delays_bytes, double epoch, void* weights, size_t weights_bytes,
+                            fbf_stream_t stream);
 
 /* ---- batched subarrays on one block (no counterpart in the reference:
  * it compiles one subarray per process, fbfuse_constants.hpp:12-14) ---- */

@@ -111,6 +111,8 @@ _SIGS = {
     "fbf_cb_beam_sums": [vp, vp, sz, vp, sz, vp, sz, vp],
     "fbf_cb_beam_power": [vp, vp, sz, vp, sz, vp, sz, vp],
     "fbf_cb_beamform_subarray": [vp, vp, sz, i32, i32, vp, sz, vp, sz, vp],
+    "fbf_cb_beamform_taftp": [vp, vp, sz, i32, i32, i32, vp, sz, vp, sz, vp],
+    "fbf_cb_taftp_direct": [vp, i32, i32, C.c_longlong, C.POINTER(C.c_int)],
     "fbf_cb_plan_set_chunk": [vp, i32],
     "fbf_cb_autotune": [vp, C.c_longlong, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)],
     "fbf_batch_create": [i32, i32, i32, i32, i32, C.POINTER(SubarrayDesc), C.POINTER(C.c_void_p),
@@ -148,6 +150,7 @@ _SIGS = {
     "fbf_ib_plan_destroy": [vp],
     "fbf_ib_output_bytes": [vp, sz, C.POINTER(C.c_size_t)],
     "fbf_ib_beamform": [vp, vp, sz, vp, sz, vp],
+    "fbf_ib_beamform_weights": [vp, vp, sz, vp, sz, vp, vp, sz, C.c_double, vp, sz, vp],
     "fbf_ib_power": [vp, vp, sz, vp, sz, vp],
     "fbf_weights_plan_create": [i32, i32, i32, vp, pvp],
     "fbf_weights_plan_destroy": [vp],
@@ -431,6 +434,23 @@ class CbPlan:
         call("fbf_cb_beamform_subarray", self.handle, ftpa_block.ptr, ftpa_block.nbytes, int(block_nantennas),
              int(antenna_offset), weights.ptr, weights.nbytes, out.ptr, out.nbytes, _s(stream))
 
+    def beamform_taftp(self, taftp, weights, out, nantennas_total=None, start_antenna=0,
+                       nsamples_per_heap=256, stream=None):
+        """Coherent beams straight from a TAFTP block (the plan's antennas
+        start at start_antenna of the block's nantennas_total)."""
+        if nantennas_total is None:
+            nantennas_total = self.desc.nantennas
+        call("fbf_cb_beamform_taftp", self.handle, taftp.ptr, taftp.nbytes, int(nantennas_total),
+             int(start_antenna), int(nsamples_per_heap), weights.ptr, weights.nbytes, out.ptr, out.nbytes,
+             _s(stream))
+
+    def taftp_direct(self, nantennas_total, nsamples, nsamples_per_heap=256):
+        """Whether beamform_taftp reads such a block without an FTPA copy."""
+        d = C.c_int()
+        call("fbf_cb_taftp_direct", self.handle, int(nantennas_total), int(nsamples_per_heap), int(nsamples),
+             C.byref(d))
+        return bool(d.value)
+
     def beam_sums(self, ftpa, weights, out, stream=None):
         call("fbf_cb_beam_sums", self.handle, ftpa.ptr, ftpa.nbytes, weights.ptr, weights.nbytes,
              out.ptr, out.nbytes, _s(stream))
@@ -470,6 +490,12 @@ class IbPlan:
 
     def beamform(self, taftp, out, stream=None):
         call("fbf_ib_beamform", self.handle, taftp.ptr, taftp.nbytes, out.ptr, out.nbytes, _s(stream))
+
+    def beamform_weights(self, taftp, out, weights_plan, delays, epoch, weights, stream=None):
+        """beamform and the weights generator in one launch."""
+        call("fbf_ib_beamform_weights", self.handle, taftp.ptr, taftp.nbytes, out.ptr, out.nbytes,
+             weights_plan.handle, delays.ptr, delays.nbytes, float(epoch), weights.ptr, weights.nbytes,
+             _s(stream))
 
     def power(self, taftp, out, stream=None):
         call("fbf_ib_power", self.handle, taftp.ptr, taftp.nbytes, out.ptr, out.nbytes, _s(stream))

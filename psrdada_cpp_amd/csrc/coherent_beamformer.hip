@@ -73,6 +73,10 @@ struct CbArgs {
     // voff = 0 for a split-transposed subset; a subarray reading a full
     // FTPA block uses the block's column stride and its antenna offset).
     int vstride, voff;
+    // S > 0: the voltages are instead a TAFTP block, [heap group][Atot][F][S
+    // samples][2 pols] char2, and this beamformer's antennas start at its
+    // antenna a0 (read so by cb_tr_k<..., TA = true> and cb_exact_outputs).
+    int S, Atot, a0;
 };
 
 // Loads of one 16-byte K slice [k0, k0 + 16) of a column of K bytes.  They
@@ -203,11 +207,23 @@ __device__ __forceinline__ void cb_exact_outputs(const CbArgs& p, int fo, int b0
         for (int fc = 0; fc < p.fs; ++fc) {
             const int f = fo * p.fs + fc;
             const int8_t* wrow = p.fba + ((size_t)f * p.B + b) * K;
-            const int8_t* col = p.ftpa + ((size_t)f * TP + (size_t)w * W) * p.vstride + p.voff;
-            for (int j = 0; j < W; ++j, col += p.vstride) {
+            for (int j = 0; j < W; ++j) {
+                // Column (sample, pol) jc of channel f; antenna a at col + a * astep.
+                const long long jc = w * W + j;
+                const int8_t* col;
+                size_t astep;
+                if (p.S > 0) {
+                    const long long t = jc >> 1;
+                    col = p.ftpa + (((size_t)(t / p.S) * p.Atot + p.a0) * p.F + f) * p.S * 4 +
+                          (size_t)(t % p.S) * 4 + 2 * (jc & 1);
+                    astep = (size_t)p.F * p.S * 4;
+                } else {
+                    col = p.ftpa + ((size_t)f * TP + (size_t)jc) * p.vstride + p.voff;
+                    astep = 2;
+                }
                 int re = 0, im = 0;
                 for (int a = 0; a < p.A; ++a) {
-                    const int vr = col[2 * a], vi = col[2 * a + 1];
+                    const int vr = col[a * astep], vi = col[a * astep + 1];
                     const int wr = wrow[2 * a], wi = wrow[2 * a + 1];
                     re += wr * vr - wi * vi;
                     im += wr * vi + wi * vr;
@@ -926,7 +942,30 @@ struct CbTrArgs {
     int npasses;               // passes per work-group
     int ntp;                   // tiles per pass
     int n_bgroups, n_tgroups;
+    // TAFTP staging: bytes per heap group of the block, log2 of the tiles
+    // (16 samples) per heap.
+    size_t hstride;
+    int lgtph;
 };
+
+// The 16-byte A operand of one K slice (8 antennas x char2 of one column)
+// from an LDS tile laid out [antenna][32 columns] int16, 64 bytes a row:
+// two ds_read_b64_tr_b16 of 4 antenna rows each.  Within every 16-lane group
+// G the instruction gives lane i, as its element r, element i % 4 of the 8
+// bytes read by lane 16 G + 4 r + i / 4.  So when lane L reads the 4
+// columns 16 (G % 2) + 4 (L % 4) .. + 3 of antenna row base + (L / 4) % 4,
+// lane c + 32 h (c < 32) receives column c of rows base .. base + 3, in
+// row order; `src` is that row piece for this lane, the second read the
+// same 4 rows further on.
+__device__ __forceinline__ v4i tr_operand(const int8_t* src)
+{
+    typedef short v4s __attribute__((ext_vector_type(4)));
+    typedef int v2i __attribute__((ext_vector_type(2)));
+    const v4s lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) v4s*)(src));
+    const v4s hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) v4s*)(src + 256));
+    const v2i l2 = __builtin_bit_cast(v2i, lo), h2 = __builtin_bit_cast(v2i, hi);
+    return v4i{l2.x, l2.y, h2.x, h2.y};
+}
 
 // a + b' where lanes below 32 keep a and take b' = (a of lane + 32), and
 // lanes from 32 keep b and take (b of lane - 32): the halves' exchange of a
@@ -1092,7 +1131,10 @@ struct TrRed {
 
 // RT: beam tiles per wave (RT / 2 groups of 32 beams, a Re and an Im tile
 // each); SPT: windows per tile (1, 2, 4); NSLOT: windows
-// per pass held in registers; FS: fscrunch; NTS: time slots per work-group.
+// per pass held in registers; FS: fscrunch; NTS: time slots per work-group;
+// TA: the voltages are a TAFTP block (CbArgs::S > 0), staged straight into
+// [antenna][column] tiles and read with transposed LDS loads (tr_operand),
+// else FTPA columns.
 //
 // A work-group has 4 * NTS waves: NTS time slots (contiguous runs of passes
 // of the same channel group and beams) x 4 waves ((4 / FS) beam groups x FS
@@ -1101,7 +1143,7 @@ struct TrRed {
 // per-step barrier keeps them in lockstep: waves of separate work-groups
 // sharing a SIMD progress at different rates (the issue arbiter prefers
 // older waves), which leaves SIMDs with one wave at the end.
-template <int RT, int KSTEPS, int SPT, int NSLOT, int FS, int NTS>
+template <int RT, int KSTEPS, int SPT, int NSLOT, int FS, int NTS, bool TA>
 __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 {
     using St = TrStage<FS, KSTEPS, NTS>;
@@ -1161,10 +1203,12 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     }
 
     // Staging.  A wave's 16-byte load lands at LDS offset 16 * lane of a
-    // 1 KiB run, so the bank swizzle is chosen on the global side: physical
-    // piece pp of column col holds K piece (pp + rot(col)) % PR with
+    // 1 KiB run.  FTPA: the bank swizzle is chosen on the global side:
+    // physical piece pp of column col holds K piece (pp + rot(col)) % PR with
     // rot(col) = col * KPAD / 256.  K pieces past the column (k >= K) re-read
-    // piece 0: the weights there are zero.
+    // piece 0: the weights there are zero.  TAFTP: a tile is 16 * KSTEPS
+    // antenna rows of 64 bytes (its 32 columns, contiguous in TAFTP), and a
+    // load fills 16 rows, 4 lanes a row; rows past A re-read antenna 0.
     const int kpieces = K / 16;
     int8_t* dummy = lds + NB * St::SET;
     long long* red = reinterpret_cast<long long*>(dummy + St::DUMMY);   // [waves][VALS][64]
@@ -1183,35 +1227,55 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
         const int u = wv + St::NW * m;
         if (u < St::NI) {
             const int ti = u / KSTEPS;
-            const int rem = (u - ti * KSTEPS) * 1024 + lane * 16;
-            const int col = rem / KPAD;
-            const int pp = (rem - col * KPAD) >> 4;
-            const int j = (pp + ((col * KPAD) >> 8)) % PR;
-            const int jj = j < kpieces ? j : 0;
-            cbase[m] = p.ftpa + (size_t)(fo * FS + ti % FS) * TP * p.vstride + p.voff;
-            goff[m] = (uint32_t)(col * p.vstride + 16 * jj);
+            if constexpr (TA) {
+                const int kp = u - ti * KSTEPS;
+                int an = 16 * kp + (lane >> 2);
+                if (an >= p.A) an = 0;
+                cbase[m] = p.ftpa + (size_t)(fo * FS + ti % FS) * p.S * 4;
+                goff[m] = (uint32_t)(p.a0 + an) * (uint32_t)(p.F * p.S * 4) + 16 * (lane & 3);
+            } else {
+                const int rem = (u - ti * KSTEPS) * 1024 + lane * 16;
+                const int col = rem / KPAD;
+                const int pp = (rem - col * KPAD) >> 4;
+                const int j = (pp + ((col * KPAD) >> 8)) % PR;
+                const int jj = j < kpieces ? j : 0;
+                cbase[m] = p.ftpa + (size_t)(fo * FS + ti % FS) * TP * p.vstride + p.voff;
+                goff[m] = (uint32_t)(col * p.vstride + 16 * jj);
+            }
             gtile[m] = (tgroup * NTS + ti / FS) * slot_tiles;
             ldst[m] = u * 1024;
         } else {
-            cbase[m] = p.ftpa + (size_t)(fo * FS) * TP * p.vstride + p.voff;
+            cbase[m] = TA ? p.ftpa : p.ftpa + (size_t)(fo * FS) * TP * p.vstride + p.voff;
             goff[m] = 0;
             gtile[m] = 0;
             ldst[m] = -1;
         }
     }
     const size_t tile_stride = (size_t)32 * p.vstride;
+    const int tmask = (1 << ta.lgtph) - 1;
     auto issue = [&](int q, int slot) {
         int8_t* set = lds + slot * St::SET;
 #pragma unroll
         for (int m = 0; m < St::M; ++m) {
             const int tt = gtile[m] + q < ta.ntiles ? gtile[m] + q : ta.ntiles - 1;
             int8_t* dst = ldst[m] >= 0 ? set + ldst[m] : dummy + wv * 1024;
-            dma16(uniform_ptr(cbase[m] + (size_t)tt * tile_stride) + goff[m], dst);
+            const size_t toff =
+                TA ? (size_t)(tt >> ta.lgtph) * ta.hstride + (size_t)(tt & tmask) * 64 : (size_t)tt * tile_stride;
+            dma16(uniform_ptr(cbase[m] + toff) + goff[m], dst);
         }
     };
-    // Physical offsets of this lane's K pieces in its column.
+    // Physical offsets of this lane's K pieces in its column (FTPA), or of
+    // the row piece this lane reads for K slice s (TAFTP, tr_operand): rows
+    // 8 (h KSTEPS + s) + (lane / 4) % 4, columns 16 ((lane / 16) % 2) +
+    // 4 (lane % 4), so lane c + 32 h gets the antennas of K piece
+    // h KSTEPS + s, as FTPA's column holds them.
     int aoff[KSTEPS];
-    {
+    if constexpr (TA) {
+        const int rr = (lane >> 2) & 3, xx = lane & 3, gh = (lane >> 4) & 1;
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s)
+            aoff[s] = (tsl * FS + wc) * KSTEPS * 1024 + (8 * (h * KSTEPS + s) + rr) * 64 + 32 * gh + 8 * xx;
+    } else {
         const int rot = (c * KPAD) >> 8;
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s)
@@ -1345,7 +1409,8 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
                     const int8_t* src = lds + rs * St::SET;
                     v4i a[KSTEPS];
 #pragma unroll
-                    for (int s = 0; s < KSTEPS; ++s) a[s] = *reinterpret_cast<const v4i*>(src + aoff[s]);
+                    for (int s = 0; s < KSTEPS; ++s)
+                        a[s] = TA ? tr_operand(src + aoff[s]) : *reinterpret_cast<const v4i*>(src + aoff[s]);
                     detect_step<RT, KSTEPS, SPT, NSLOT>(a, w, pw, g);
                 }
                 rs = rs + 1 == NB ? 0 : rs + 1;
@@ -1457,15 +1522,15 @@ inline int device_cu_count()
 // contiguous run of passes, so the per-work-group start-up (weights, the
 // first staged tiles) is paid once per resident slot and all work-groups end
 // together.
-template <int RT, int KSTEPS, int SPT, int NSLOT, int FS>
-hipError_t launch_tr_k(CbTrArgs a, int passes, hipStream_t s)
+template <int RT, int KSTEPS, int SPT, int NSLOT, int FS, bool TA>
+hipError_t launch_tr_kt(CbTrArgs a, int passes, hipStream_t s)
 {
     constexpr int NTS = tr_nts(RT, NSLOT);
     const size_t shm = cb_tr_shm(FS, KSTEPS, TrRed<RT, NSLOT>::VALS, NTS);
     static int occ = 0;
     if (occ == 0) {
         int o = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, cb_tr_k<RT, KSTEPS, SPT, NSLOT, FS, NTS>, 256 * NTS,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, cb_tr_k<RT, KSTEPS, SPT, NSLOT, FS, NTS, TA>, 256 * NTS,
                                                          shm) != hipSuccess ||
             o < 1)
             o = 1;
@@ -1483,8 +1548,21 @@ hipError_t launch_tr_k(CbTrArgs a, int passes, hipStream_t s)
     a.n_tgroups = (int)((passes + (long long)a.npasses * NTS - 1) / ((long long)a.npasses * NTS));
     const long long nblocks = units * a.n_tgroups;
     if (nblocks <= 0 || nblocks >= (1ll << 31)) return hipErrorInvalidConfiguration;
-    cb_tr_k<RT, KSTEPS, SPT, NSLOT, FS, NTS><<<dim3((unsigned)nblocks), 256 * NTS, shm, s>>>(a);
+    cb_tr_k<RT, KSTEPS, SPT, NSLOT, FS, NTS, TA><<<dim3((unsigned)nblocks), 256 * NTS, shm, s>>>(a);
     return hipGetLastError();
+}
+
+// TAFTP staging (a.p.S > 0) is built for 64 and 128 beams per wave and
+// fscrunch 1 only (cb_taftp_direct): with 64 beams per wave no fscrunch above
+// 1 fits the kernel's LDS (cb_tr_supported).
+template <int RT, int KSTEPS, int SPT, int NSLOT, int FS>
+hipError_t launch_tr_k(CbTrArgs a, int passes, hipStream_t s)
+{
+    if (a.p.S > 0) {
+        if constexpr (RT != 2 && FS == 1) return launch_tr_kt<RT, KSTEPS, SPT, NSLOT, FS, true>(a, passes, s);
+        return hipErrorInvalidValue;
+    }
+    return launch_tr_kt<RT, KSTEPS, SPT, NSLOT, FS, false>(a, passes, s);
 }
 
 template <int RT, int SPT, int NSLOT, int FS>
@@ -2131,6 +2209,8 @@ struct fbf_cb_plan {
     float* d_levels;  // device: offsets[Fo], scales[Fo], then the thresholds [Fo][kThr] (int64)
     void* staging;    // [Fo][To][B] int8 between the beamformer and tbtf_k
     size_t staging_bytes;
+    void* ftpa;       // FTPA of fbf_cb_beamform_taftp's split-transpose path
+    size_t ftpa_bytes;
 };
 
 static long long* cb_thr(const fbf_cb_plan* plan)
@@ -2138,26 +2218,32 @@ static long long* cb_thr(const fbf_cb_plan* plan)
     return reinterpret_cast<long long*>(plan->d_levels + 2 * (plan->d.nchans / plan->d.fscrunch));
 }
 
-// Grows the plan's staging buffer (not while the stream is being captured).
-static int cb_staging(fbf_cb_plan* plan, size_t bytes, hipStream_t s, void** out)
+// Grows one of the plan's device buffers (not while the stream is being
+// captured).
+static int cb_grow(void** buf, size_t* have, size_t bytes, hipStream_t s, void** out)
 {
-    if (plan->staging_bytes < bytes) {
+    if (*have < bytes) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         FBF_HIP_TRY(hipStreamIsCapturing(s, &cs));
         FBF_REQUIRE(cs == hipStreamCaptureStatusNone, FBF_ERR_STATE,
-                    "fbf_cb: the plan's staging buffer must be sized (fbf_cb_plan_reserve) before stream capture");
+                    "fbf_cb: the plan's buffers must be sized (fbf_cb_plan_reserve) before stream capture");
         // Earlier launches on any stream may still use the old buffer.
-        if (plan->staging) {
+        if (*buf) {
             FBF_HIP_TRY(hipDeviceSynchronize());
-            FBF_HIP_TRY(hipFree(plan->staging));
+            FBF_HIP_TRY(hipFree(*buf));
         }
-        plan->staging = nullptr;
-        plan->staging_bytes = 0;
-        FBF_HIP_TRY(hipMalloc(&plan->staging, bytes));
-        plan->staging_bytes = bytes;
+        *buf = nullptr;
+        *have = 0;
+        FBF_HIP_TRY(hipMalloc(buf, bytes));
+        *have = bytes;
     }
-    *out = plan->staging;
+    *out = *buf;
     return FBF_OK;
+}
+
+static int cb_staging(fbf_cb_plan* plan, size_t bytes, hipStream_t s, void** out)
+{
+    return cb_grow(&plan->staging, &plan->staging_bytes, bytes, s, out);
 }
 
 static int cb_validate(const fbf_cb_desc* d)
@@ -2198,7 +2284,7 @@ extern "C" int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets,
     const int Fo = desc->nchans / desc->fscrunch;
     int ohs = desc->nsamples_per_heap_out ? desc->nsamples_per_heap_out : 8192 / Fo;
     FBF_REQUIRE(ohs > 0, FBF_ERR_ARG, "fbf_cb_plan_create: output heap holds no samples");
-    auto* p = new fbf_cb_plan{*desc, ohs, 0, 0, nullptr, nullptr, 0};
+    auto* p = new fbf_cb_plan{*desc, ohs, 0, 0, nullptr, nullptr, 0, nullptr, 0};
     hipError_t e = hipMalloc(&p->d_levels, sizeof(float) * 2 * Fo + sizeof(long long) * kThr * Fo);
     if (e != hipSuccess) {
         delete p;
@@ -2245,6 +2331,10 @@ extern "C" int fbf_cb_plan_destroy(fbf_cb_plan* plan)
     hipError_t e = hipFree(plan->d_levels);
     if (plan->staging) {
         const hipError_t e2 = hipFree(plan->staging);
+        if (e == hipSuccess) e = e2;
+    }
+    if (plan->ftpa) {
+        const hipError_t e2 = hipFree(plan->ftpa);
         if (e == hipSuccess) e = e2;
     }
     delete plan;
@@ -2349,12 +2439,19 @@ extern "C" int fbf_cb_output_bytes(const fbf_cb_plan* plan, size_t ftpa_bytes, s
     return FBF_OK;
 }
 
+// A TAFTP block read directly by the coherent beamformer: S samples per
+// heap, Atot antennas, the beamformer's antennas from antenna a0.
+struct TaftpView {
+    int S, Atot, a0;
+};
+
 // Validates and launches one coherent-beam kernel over T samples of voltage
 // columns that are `vstride` bytes apart, the beamformer's antennas starting
-// `voff` bytes into each column.
+// `voff` bytes into each column; or, with `tv`, over the TAFTP block `ftpa`
+// (the caller has checked cb_taftp_direct).
 static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T, int vstride, int voff,
                      const void* fba, size_t weights_bytes, void* out, size_t out_bytes, fbf_stream_t stream,
-                     void* workspace = nullptr, size_t workspace_bytes = 0)
+                     void* workspace = nullptr, size_t workspace_bytes = 0, const TaftpView* tv = nullptr)
 {
     FBF_REQUIRE(plan, FBF_ERR_ARG, "fbf_cb: null plan");
     FBF_REQUIRE(ftpa && fba && out, FBF_ERR_ARG, "fbf_cb: null buffer");
@@ -2400,6 +2497,8 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
     }
 
     const CbVariant var = cb_pick_variant(d, plan->variant, mode, T);
+    FBF_REQUIRE(!tv || (var.tr && !var.ws && var.rows != 2 && d.fscrunch == 1), FBF_ERR_STATE,
+                "fbf_cb: the TAFTP input needs the transposed kernel with 64 or 128 beams per wave, fscrunch 1");
     if (var.ws) {
         CbWsArgs wa{};
         CbArgs& a = wa.p;
@@ -2457,6 +2556,13 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
         a.quant = d.quant_mode;
         a.vstride = vstride;
         a.voff = voff;
+        if (tv) {
+            a.S = tv->S;
+            a.Atot = tv->Atot;
+            a.a0 = tv->a0;
+            ta.hstride = (size_t)4 * tv->Atot * d.nchans * tv->S;
+            ta.lgtph = __builtin_ctz((unsigned)(tv->S / 16));
+        }
         const int rt = var.rows;
         const int W = 2 * d.tscrunch;
         const int spt = W >= 32 ? 1 : 32 / W;
@@ -2491,7 +2597,7 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
                     "fbf_cb: chunk must divide 2*tscrunch and fit the prefetch registers");
         jc = plan->chunk;
     }
-    CbArgs a;
+    CbArgs a{};
     a.ftpa = static_cast<const int8_t*>(ftpa);
     a.fba = static_cast<const int8_t*>(fba);
     a.out = kout;
@@ -2583,6 +2689,70 @@ extern "C" int fbf_cb_beamform_subarray(fbf_cb_plan* plan, const void* ftpa_bloc
     const long long T = (long long)(block_bytes / per_sample);
     return cb_launch(plan, MODE_INT8, ftpa_block, T, 2 * block_nantennas, 2 * antenna_offset, fba_weights,
                      weights_bytes, tbtf, tbtf_bytes, stream);
+}
+
+// Whether fbf_cb_beamform_taftp stages the TAFTP block straight into the
+// transposed kernel: the plan's kernel is cb_tr_k with 64 or 128 beams per
+// wave and fscrunch 1, a heap holds a power-of-two number of samples, at least one tile
+// (16 samples), and a heap group's bytes fit a 31-bit offset.
+static bool cb_taftp_direct(const fbf_cb_plan* plan, long long T, int nantennas_total, int S)
+{
+    const fbf_cb_desc& d = plan->d;
+    const CbVariant var = cb_pick_variant(d, plan->variant, MODE_INT8, T);
+    return var.tr && !var.ws && var.rows != 2 && d.fscrunch == 1 && S >= 16 && (S & (S - 1)) == 0 &&
+           (long long)4 * nantennas_total * d.nchans * S < (1ll << 31);
+}
+
+extern "C" int fbf_cb_beamform_taftp(fbf_cb_plan* plan, const void* taftp, size_t taftp_bytes, int nantennas_total,
+                                     int start_antenna, int nsamples_per_heap, const void* fba_weights,
+                                     size_t weights_bytes, void* tbtf, size_t tbtf_bytes, fbf_stream_t stream)
+{
+    FBF_REQUIRE(plan, FBF_ERR_ARG, "fbf_cb_beamform_taftp: null plan");
+    FBF_REQUIRE(taftp && fba_weights && tbtf, FBF_ERR_ARG, "fbf_cb_beamform_taftp: null buffer");
+    const fbf_cb_desc& d = plan->d;
+    FBF_REQUIRE(nantennas_total > 0 && start_antenna >= 0 && start_antenna + d.nantennas <= nantennas_total,
+                FBF_ERR_ARG, "fbf_cb_beamform_taftp: the plan's antennas lie outside the block's");
+    FBF_REQUIRE(nsamples_per_heap > 0, FBF_ERR_ARG, "fbf_cb_beamform_taftp: nsamples_per_heap must be positive");
+    const size_t hg_bytes = (size_t)4 * nantennas_total * d.nchans * nsamples_per_heap;
+    FBF_REQUIRE(taftp_bytes % hg_bytes == 0, FBF_ERR_SIZE,
+                "fbf_cb_beamform_taftp: input is not a whole number of heap groups");
+    const long long T = (long long)(taftp_bytes / hg_bytes) * nsamples_per_heap;
+    FBF_REQUIRE((reinterpret_cast<uintptr_t>(taftp) & 15) == 0, FBF_ERR_ARG,
+                "fbf_cb_beamform_taftp: buffers must be 16-byte aligned");
+    if (T == 0 || cb_taftp_direct(plan, T, nantennas_total, nsamples_per_heap)) {
+        const TaftpView tv{nsamples_per_heap, nantennas_total, start_antenna};
+        return cb_launch(plan, MODE_INT8, taftp, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf,
+                         tbtf_bytes, stream, nullptr, 0, T ? &tv : nullptr);
+    }
+    // Otherwise the plan's antennas are split-transposed into the plan's FTPA
+    // buffer for the FTPA kernels.  Sizes are checked before any launch.
+    const size_t fbytes = (size_t)4 * d.nantennas * d.nchans * T;
+    FBF_REQUIRE(weights_bytes == (size_t)2 * d.nchans * d.nbeams * d.nantennas, FBF_ERR_SIZE,
+                "fbf_cb_beamform_taftp: weights size " + std::to_string(weights_bytes) + " != expected " +
+                    std::to_string((size_t)2 * d.nchans * d.nbeams * d.nantennas));
+    size_t expect = 0;
+    int st = fbf_cb_output_bytes(plan, fbytes, &expect);
+    if (st) return st;
+    FBF_REQUIRE(tbtf_bytes == expect && (T / d.tscrunch) % plan->ohs == 0, FBF_ERR_SIZE,
+                "fbf_cb_beamform_taftp: output size " + std::to_string(tbtf_bytes) + " != expected " +
+                    std::to_string(expect) + ", or not whole output heaps");
+    void* ftpa = nullptr;
+    st = cb_grow(&plan->ftpa, &plan->ftpa_bytes, fbytes, static_cast<hipStream_t>(stream), &ftpa);
+    if (st) return st;
+    st = fbf_split_transpose(taftp, taftp_bytes, ftpa, fbytes, nantennas_total, d.nantennas, start_antenna,
+                             d.nchans, nsamples_per_heap, stream);
+    if (st) return st;
+    return cb_launch(plan, MODE_INT8, ftpa, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf, tbtf_bytes,
+                     stream);
+}
+
+extern "C" int fbf_cb_taftp_direct(const fbf_cb_plan* plan, int nantennas_total, int nsamples_per_heap,
+                                   long long nsamples, int* direct)
+{
+    FBF_REQUIRE(plan && direct, FBF_ERR_ARG, "fbf_cb_taftp_direct: null argument");
+    *direct = nsamples > 0 && nsamples % plan->d.tscrunch == 0 &&
+              cb_taftp_direct(plan, nsamples, nantennas_total, nsamples_per_heap);
+    return FBF_OK;
 }
 
 extern "C" int fbf_cb_plan_set_chunk(fbf_cb_plan* plan, int chunk)

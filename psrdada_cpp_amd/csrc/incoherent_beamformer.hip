@@ -13,6 +13,9 @@
 // every load instruction is fully coalesced, and squares with v_dot4_i32_i8
 // (exact int32).  tscrunch is reduced in-lane and across lanes with
 // shuffles, fscrunch across rows through LDS, in int64 (exact).
+//
+// fbf_ib_beamform_weights also generates the block's beamforming weights in
+// the same launch (its first work-groups), as the split-transpose launch can.
 #include "fbf_internal.hpp"
 
 namespace fbf {
@@ -34,14 +37,15 @@ __device__ __forceinline__ int sq4(int x, int acc)
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void ib_k(IbArgs p)
+__global__ __launch_bounds__(256) void ib_k(IbArgs p, WeightsJob wj)
 {
     constexpr int S = 256;
     __shared__ long long part[4][S];
     const int tid = threadIdx.x;
+    if (weights_job_run(wj, tid)) return;   // uniform over the work-group
     const int sc = tid & 63;   // 16-byte chunk: samples 4*sc .. 4*sc+3
     const int row = tid >> 6;  // 0..3
-    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int logical = xcd_remap((int)blockIdx.x - wj.nblocks, (int)gridDim.x - wj.nblocks);
     const int fb = logical % p.nfb;
     const int hg = logical / p.nfb;
     if (hg >= p.nhg) return;
@@ -207,7 +211,7 @@ extern "C" int fbf_ib_output_bytes(const fbf_ib_plan* plan, size_t taftp_bytes, 
 }
 
 static int ib_run(fbf_ib_plan* plan, int mode, const void* taftp, size_t taftp_bytes, void* out, size_t out_bytes,
-                  fbf_stream_t stream)
+                  fbf_stream_t stream, WeightsJob wj = WeightsJob{})
 {
     FBF_REQUIRE(plan && taftp && out, FBF_ERR_ARG, "fbf_ib: null argument");
     long long nhg = 0;
@@ -220,7 +224,7 @@ static int ib_run(fbf_ib_plan* plan, int mode, const void* taftp, size_t taftp_b
     FBF_REQUIRE(out_bytes == expect, FBF_ERR_SIZE,
                 "fbf_ib: output size " + std::to_string(out_bytes) + " != expected " + std::to_string(expect));
     FBF_REQUIRE((reinterpret_cast<uintptr_t>(taftp) & 15) == 0, FBF_ERR_ARG, "fbf_ib: input must be 16-byte aligned");
-    if (nhg == 0) return FBF_OK;
+    if (nhg == 0 && wj.nblocks == 0) return FBF_OK;
     IbArgs a;
     a.taftp = static_cast<const int8_t*>(taftp);
     a.out = out;
@@ -235,13 +239,13 @@ static int ib_run(fbf_ib_plan* plan, int mode, const void* taftp, size_t taftp_b
     a.quant = d.quant_mode;
     a.cpb = d.fscrunch > 4 ? d.fscrunch : 4;
     a.nfb = d.nchans / a.cpb;
-    const long long nblocks = nhg * a.nfb;
+    const long long nblocks = nhg * a.nfb + wj.nblocks;
     FBF_REQUIRE(nblocks < (1ll << 31), FBF_ERR_UNSUPPORTED, "fbf_ib: grid too large");
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (mode == IB_INT8)
-        ib_k<IB_INT8><<<dim3((unsigned)nblocks), 256, 0, s>>>(a);
+        ib_k<IB_INT8><<<dim3((unsigned)nblocks), 256, 0, s>>>(a, wj);
     else
-        ib_k<IB_POWER><<<dim3((unsigned)nblocks), 256, 0, s>>>(a);
+        ib_k<IB_POWER><<<dim3((unsigned)nblocks), 256, 0, s>>>(a, wj);
     FBF_LAUNCH_CHECK("ib_k");
     return FBF_OK;
 }
@@ -265,6 +269,23 @@ extern "C" int fbf_ib_beamform(fbf_ib_plan* plan, const void* taftp, size_t taft
                                fbf_stream_t stream)
 {
     return ib_run(plan, IB_INT8, taftp, taftp_bytes, tf, tf_bytes, stream);
+}
+
+extern "C" int fbf_ib_beamform_weights(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes, void* tf,
+                                       size_t tf_bytes, fbf_weights_plan* weights_plan, const void* delays,
+                                       size_t delays_bytes, double epoch, void* weights, size_t weights_bytes,
+                                       fbf_stream_t stream)
+{
+    FBF_REQUIRE(plan, FBF_ERR_ARG, "fbf_ib_beamform_weights: null plan");
+    WeightsJob wj{};
+    int st = weights_launch_params(weights_plan, delays, delays_bytes, epoch, weights, weights_bytes, &wj.w);
+    if (st) return st;
+    FBF_REQUIRE(wj.w.F == plan->d.nchans, FBF_ERR_ARG,
+                "fbf_ib_beamform_weights: the weights plan's channels differ from the block's");
+    FBF_REQUIRE((unsigned long long)wj.w.gx * wj.w.gy < (1ull << 30), FBF_ERR_UNSUPPORTED,
+                "fbf_ib_beamform_weights: too many weights work-groups");
+    wj.nblocks = (int)(wj.w.gx * wj.w.gy);
+    return ib_run(plan, IB_INT8, taftp, taftp_bytes, tf, tf_bytes, stream, wj);
 }
 
 extern "C" int fbf_ib_power(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes, double* power,

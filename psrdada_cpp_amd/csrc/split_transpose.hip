@@ -41,13 +41,6 @@ struct IbTile {
     int tscrunch, quant;
 };
 
-// Weights generated by the first nblocks work-groups of a transpose launch
-// (nblocks = 0: none).
-struct WeightsJob {
-    WeightsLaunch w;
-    int nblocks;   // w.gx * w.gy
-};
-
 template <int U, bool IB>
 __global__ __launch_bounds__(256) void split_transpose_k(
     const int16_t* __restrict__ in, int16_t* __restrict__ out,
@@ -57,12 +50,7 @@ __global__ __launch_bounds__(256) void split_transpose_k(
     __shared__ int16_t tile[kTileA][kTileSP + kPad];
     __shared__ int ibsum[4][kTileSP / 2];
     const int tid = threadIdx.x;
-    if ((int)blockIdx.x < wj.nblocks) {   // uniform over the work-group
-        const int wb = blockIdx.x;
-        const int di = (wb % (int)wj.w.gx) * 256 + tid;
-        if (di < wj.w.nd) weights_run(wj.w, nullptr, di, (wb / (int)wj.w.gx) * kWChans);
-        return;
-    }
+    if (weights_job_run(wj, tid)) return;   // uniform over the work-group
     const int logical = xcd_remap((int)blockIdx.x - wj.nblocks, (int)gridDim.x - wj.nblocks);
     const int spt = logical % n_sp_tiles;
     const int rest = logical / n_sp_tiles;

@@ -191,7 +191,10 @@ int fbf_cb_beamform_subarray(fbf_cb_plan* plan, const void* ftpa_block, size_t b
  * Where fbf_cb_taftp_direct says so, the kernel stages the block's
  * [antenna][sample, pol] runs itself and transposes them in LDS, so no FTPA
  * copy is written; otherwise the call split-transposes into an FTPA buffer
- * that the plan owns (grown on demand, as the staging buffer) first. */
+ * that the plan owns (grown on demand, as the staging buffer) first.  With
+ * the automatic variant and more than 48 antennas it picks the transposed
+ * kernel with 128 beams per wave where that applies (half the transposed
+ * LDS reads per MFMA of 64 beams per wave). */
 int fbf_cb_beamform_taftp(fbf_cb_plan* plan, const void* taftp, size_t taftp_bytes,
                           int nantennas_total, int start_antenna, int nsamples_per_heap,
                           const void* fba_weights, size_t weights_bytes,

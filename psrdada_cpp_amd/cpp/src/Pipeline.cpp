@@ -135,6 +135,23 @@ void Pipeline::init(RawBytes& header)
 void Pipeline::process(VoltageVectorType const& taftp, PowerVectorType& tbtf, PowerVectorType& tf)
 {
     auto const& delays = _delay_manager->delays();
+    int direct = 0;
+    FBF_ERROR_CHECK(fbf_cb_taftp_direct(_cb_plan, (int)_config.total_nantennas(), (int)_config.nsamples_per_heap(),
+                                        (long long)_nsamples_per_dada_block, &direct));
+    if (direct) {
+        // The weights in the incoherent beam's launch, then the coherent
+        // beams straight from the TAFTP block: no split-transpose pass, the
+        // same outputs as the reference's order (Pipeline.cu:164-170).
+        FBF_ERROR_CHECK(fbf_ib_beamform_weights(_ib_plan, taftp.data(), taftp.bytes(), tf.data(), tf.bytes(),
+                                                _weights_plan, delays.data(), delays.bytes(),
+                                                (double)_unix_timestamp, _weights.data(), _weights.bytes(),
+                                                _processing_stream));
+        FBF_ERROR_CHECK(fbf_cb_beamform_taftp(_cb_plan, taftp.data(), taftp.bytes(), (int)_config.total_nantennas(),
+                                              (int)_config.cb_antenna_offset(), (int)_config.nsamples_per_heap(),
+                                              _weights.data(), _weights.bytes(), tbtf.data(), tbtf.bytes(),
+                                              _processing_stream));
+        return;
+    }
     FBF_ERROR_CHECK(fbf_weights_generate(_weights_plan, delays.data(), delays.bytes(), (double)_unix_timestamp,
                                          _weights.data(), _weights.bytes(), nullptr, _processing_stream));
     _split_transpose_output.resize(_nsamples_per_dada_block * _config.npol() * _config.nchans() *

@@ -948,23 +948,78 @@ struct CbTrArgs {
     int lgtph;
 };
 
-// The 16-byte A operand of one K slice (8 antennas x char2 of one column)
-// from an LDS tile laid out [antenna][32 columns] int16, 64 bytes a row:
-// two ds_read_b64_tr_b16 of 4 antenna rows each.  Within every 16-lane group
-// G the instruction gives lane i, as its element r, element i % 4 of the 8
-// bytes read by lane 16 G + 4 r + i / 4.  So when lane L reads the 4
-// columns 16 (G % 2) + 4 (L % 4) .. + 3 of antenna row base + (L / 4) % 4,
-// lane c + 32 h (c < 32) receives column c of rows base .. base + 3, in
-// row order; `src` is that row piece for this lane, the second read the
-// same 4 rows further on.
-__device__ __forceinline__ v4i tr_operand(const int8_t* src)
+// The 16-byte A operands of one step's K slices (8 antennas x char2 of one
+// column each) from an LDS tile laid out [antenna][32 columns] int16, 64
+// bytes a row: two ds_read_b64_tr_b16 of 4 antenna rows each per slice.
+// Within every 16-lane group G the instruction gives lane i, as its element
+// r, element i % 4 of the 8 bytes read by lane 16 G + 4 r + i / 4.  So when
+// lane L reads the 4 columns 16 (G % 2) + 4 (L % 4) .. + 3 of antenna row
+// base + (L / 4) % 4, lane c + 32 h (c < 32) receives column c of rows
+// base .. base + 3, in row order.  `src` is that row piece of slice 0 for
+// this lane; slice s is 8 rows (512 bytes) further on, its second read 4
+// rows (256 bytes) after its first.
+//
+// All the reads are issued in one asm block that ends with their LDS wait.
+// Issued through the builtin, every transposed read is preceded by a wait
+// for all outstanding global->LDS loads (the compiler cannot tell that they
+// fill other ring slots), which cost the kernel a memory round trip per
+// step; the ring's own vmcnt wait and barrier already order them.  The
+// outputs are complete when the block ends (lgkmcnt(0) inside it).
+template <int KSTEPS>
+__device__ __forceinline__ void tr_operands(const int8_t* src, v4i (&a)[KSTEPS])
 {
-    typedef short v4s __attribute__((ext_vector_type(4)));
     typedef int v2i __attribute__((ext_vector_type(2)));
-    const v4s lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) v4s*)(src));
-    const v4s hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) v4s*)(src + 256));
-    const v2i l2 = __builtin_bit_cast(v2i, lo), h2 = __builtin_bit_cast(v2i, hi);
-    return v4i{l2.x, l2.y, h2.x, h2.y};
+    const uint32_t addr = (uint32_t)(size_t)(const __attribute__((address_space(3))) int8_t*)src;
+    v2i r[2 * KSTEPS];
+    static_assert(KSTEPS >= 1 && KSTEPS <= 4, "K steps");
+    if constexpr (KSTEPS == 1) {
+        __asm__ volatile(
+            "ds_read_b64_tr_b16 %0, %2\n\t"
+            "ds_read_b64_tr_b16 %1, %2 offset:256\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(r[0]), "=&v"(r[1])
+            : "v"(addr)
+            : "memory");
+    } else if constexpr (KSTEPS == 2) {
+        __asm__ volatile(
+            "ds_read_b64_tr_b16 %0, %4\n\t"
+            "ds_read_b64_tr_b16 %1, %4 offset:256\n\t"
+            "ds_read_b64_tr_b16 %2, %4 offset:512\n\t"
+            "ds_read_b64_tr_b16 %3, %4 offset:768\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3])
+            : "v"(addr)
+            : "memory");
+    } else if constexpr (KSTEPS == 3) {
+        __asm__ volatile(
+            "ds_read_b64_tr_b16 %0, %6\n\t"
+            "ds_read_b64_tr_b16 %1, %6 offset:256\n\t"
+            "ds_read_b64_tr_b16 %2, %6 offset:512\n\t"
+            "ds_read_b64_tr_b16 %3, %6 offset:768\n\t"
+            "ds_read_b64_tr_b16 %4, %6 offset:1024\n\t"
+            "ds_read_b64_tr_b16 %5, %6 offset:1280\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4]), "=&v"(r[5])
+            : "v"(addr)
+            : "memory");
+    } else {
+        __asm__ volatile(
+            "ds_read_b64_tr_b16 %0, %8\n\t"
+            "ds_read_b64_tr_b16 %1, %8 offset:256\n\t"
+            "ds_read_b64_tr_b16 %2, %8 offset:512\n\t"
+            "ds_read_b64_tr_b16 %3, %8 offset:768\n\t"
+            "ds_read_b64_tr_b16 %4, %8 offset:1024\n\t"
+            "ds_read_b64_tr_b16 %5, %8 offset:1280\n\t"
+            "ds_read_b64_tr_b16 %6, %8 offset:1536\n\t"
+            "ds_read_b64_tr_b16 %7, %8 offset:1792\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4]), "=&v"(r[5]), "=&v"(r[6]),
+              "=&v"(r[7])
+            : "v"(addr)
+            : "memory");
+    }
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) a[s] = v4i{r[2 * s].x, r[2 * s].y, r[2 * s + 1].x, r[2 * s + 1].y};
 }
 
 // a + b' where lanes below 32 keep a and take b' = (a of lane + 32), and
@@ -1133,7 +1188,7 @@ struct TrRed {
 // each); SPT: windows per tile (1, 2, 4); NSLOT: windows
 // per pass held in registers; FS: fscrunch; NTS: time slots per work-group;
 // TA: the voltages are a TAFTP block (CbArgs::S > 0), staged straight into
-// [antenna][column] tiles and read with transposed LDS loads (tr_operand),
+// [antenna][column] tiles and read with transposed LDS loads (tr_operands),
 // else FTPA columns.
 //
 // A work-group has 4 * NTS waves: NTS time slots (contiguous runs of passes
@@ -1265,7 +1320,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
         }
     };
     // Physical offsets of this lane's K pieces in its column (FTPA), or of
-    // the row piece this lane reads for K slice s (TAFTP, tr_operand): rows
+    // the row piece this lane reads for K slice s (TAFTP, tr_operands): rows
     // 8 (h KSTEPS + s) + (lane / 4) % 4, columns 16 ((lane / 16) % 2) +
     // 4 (lane % 4), so lane c + 32 h gets the antennas of K piece
     // h KSTEPS + s, as FTPA's column holds them.
@@ -1408,9 +1463,12 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
                 if (live) {
                     const int8_t* src = lds + rs * St::SET;
                     v4i a[KSTEPS];
+                    if constexpr (TA) {
+                        tr_operands<KSTEPS>(src + aoff[0], a);
+                    } else {
 #pragma unroll
-                    for (int s = 0; s < KSTEPS; ++s)
-                        a[s] = TA ? tr_operand(src + aoff[s]) : *reinterpret_cast<const v4i*>(src + aoff[s]);
+                        for (int s = 0; s < KSTEPS; ++s) a[s] = *reinterpret_cast<const v4i*>(src + aoff[s]);
+                    }
                     detect_step<RT, KSTEPS, SPT, NSLOT>(a, w, pw, g);
                 }
                 rs = rs + 1 == NB ? 0 : rs + 1;
@@ -2369,6 +2427,7 @@ constexpr int kCbVariantLds = 5;
 constexpr int kCbVariantTr2 = 8;
 constexpr int kCbVariantTr4 = 9;
 constexpr int kCbVariantWs = 10;
+constexpr int kCbVariantTr8 = 11;
 
 static bool cb_tr_applies(const fbf_cb_desc& d, int rt, int mode, long long T)
 {
@@ -2443,6 +2502,7 @@ extern "C" int fbf_cb_output_bytes(const fbf_cb_plan* plan, size_t ftpa_bytes, s
 // heap, Atot antennas, the beamformer's antennas from antenna a0.
 struct TaftpView {
     int S, Atot, a0;
+    int variant;   // the kernel variant to pick from (cb_taftp_variant)
 };
 
 // Validates and launches one coherent-beam kernel over T samples of voltage
@@ -2496,7 +2556,7 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
         }
     }
 
-    const CbVariant var = cb_pick_variant(d, plan->variant, mode, T);
+    const CbVariant var = cb_pick_variant(d, tv ? tv->variant : plan->variant, mode, T);
     FBF_REQUIRE(!tv || (var.tr && !var.ws && var.rows != 2 && d.fscrunch == 1), FBF_ERR_STATE,
                 "fbf_cb: the TAFTP input needs the transposed kernel with 64 or 128 beams per wave, fscrunch 1");
     if (var.ws) {
@@ -2695,10 +2755,25 @@ extern "C" int fbf_cb_beamform_subarray(fbf_cb_plan* plan, const void* ftpa_bloc
 // transposed kernel: the plan's kernel is cb_tr_k with 64 or 128 beams per
 // wave and fscrunch 1, a heap holds a power-of-two number of samples, at least one tile
 // (16 samples), and a heap group's bytes fit a 31-bit offset.
+// The variant fbf_cb_beamform_taftp picks from: the plan's; when automatic
+// and the FTPA pick would be the transposed kernel (more than three K steps),
+// 128 beams per wave where that applies.  That reads each staged tile once
+// per 128 beams, which halves the transposed LDS reads per MFMA against 64
+// beams per wave (134.3 us against 137.3 us at 64 antennas, 64 channels,
+// 8192 samples, 512 beams, tscrunch 16).
+static int cb_taftp_variant(const fbf_cb_plan* plan, long long T)
+{
+    const fbf_cb_desc& d = plan->d;
+    if (plan->variant > 0) return plan->variant;
+    if ((2 * d.nantennas + 31) / 32 > 3 && d.fscrunch == 1 && cb_tr_applies(d, 8, MODE_INT8, T))
+        return kCbVariantTr8;
+    return 0;
+}
+
 static bool cb_taftp_direct(const fbf_cb_plan* plan, long long T, int nantennas_total, int S)
 {
     const fbf_cb_desc& d = plan->d;
-    const CbVariant var = cb_pick_variant(d, plan->variant, MODE_INT8, T);
+    const CbVariant var = cb_pick_variant(d, cb_taftp_variant(plan, T), MODE_INT8, T);
     return var.tr && !var.ws && var.rows != 2 && d.fscrunch == 1 && S >= 16 && (S & (S - 1)) == 0 &&
            (long long)4 * nantennas_total * d.nchans * S < (1ll << 31);
 }
@@ -2720,7 +2795,7 @@ extern "C" int fbf_cb_beamform_taftp(fbf_cb_plan* plan, const void* taftp, size_
     FBF_REQUIRE((reinterpret_cast<uintptr_t>(taftp) & 15) == 0, FBF_ERR_ARG,
                 "fbf_cb_beamform_taftp: buffers must be 16-byte aligned");
     if (T == 0 || cb_taftp_direct(plan, T, nantennas_total, nsamples_per_heap)) {
-        const TaftpView tv{nsamples_per_heap, nantennas_total, start_antenna};
+        const TaftpView tv{nsamples_per_heap, nantennas_total, start_antenna, cb_taftp_variant(plan, T)};
         return cb_launch(plan, MODE_INT8, taftp, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf,
                          tbtf_bytes, stream, nullptr, 0, T ? &tv : nullptr);
     }

@@ -566,7 +566,11 @@ void pipeline_simple_run()
 
 // The beams leaving the pipeline are those of the operators for the same
 // block (block n in, its beams out on call n + 2), with the delay model read
-// from shared memory and weights at each block's unix time.
+// from shared memory and weights at each block's unix time: coherent beams
+// through SplitTranspose + CoherentBeamformer, the incoherent beam through
+// IncoherentBeamformer.  At 16 antennas the pipeline split-transposes (the
+// reference's order); at 64 it forms the coherent beams straight from the
+// TAFTP block with the weights in the incoherent beam's launch.
 struct CaptureSink {
     std::vector<std::vector<int8_t>> blocks;
     std::string header;
@@ -578,13 +582,13 @@ struct CaptureSink {
     }
 };
 
-void pipeline_outputs_match_operators()
+void pipeline_outputs_match_operators(int A, int B)
 {
     PipelineConfig config;
-    config.total_nantennas(16);
+    config.total_nantennas(A);
     config.nchans(16);
-    config.cb_nantennas(16);
-    config.cb_nbeams(64);
+    config.cb_nantennas(A);
+    config.cb_nbeams(B);
     config.cb_fscrunch(1);
     config.cb_tscrunch(16);
     config.ib_fscrunch(1);
@@ -604,8 +608,8 @@ void pipeline_outputs_match_operators()
     simulator.update_delays();
 
     const int nhg = 8;
-    const std::size_t block_bytes = (std::size_t)nhg * 16 * 16 * 256 * 2 * 2;
-    const std::size_t cb_bytes = 64 * (nhg * 256 / 16) * 16;
+    const std::size_t block_bytes = (std::size_t)nhg * A * 16 * 256 * 2 * 2;
+    const std::size_t cb_bytes = (std::size_t)B * (nhg * 256 / 16) * 16;
     const std::size_t ib_bytes = (nhg * 256 / 16) * 16;
     DadaDB cb_buffer(16, cb_bytes, 4, 4096), ib_buffer(16, ib_bytes, 4, 4096);
     cb_buffer.create();
@@ -615,7 +619,7 @@ void pipeline_outputs_match_operators()
     const int nblocks = 6;
     std::vector<std::vector<char2>> inputs;
     for (int i = 0; i < nblocks; ++i) inputs.push_back(gaussian_char2(5000 + i, 16.0, block_bytes / 2, true));
-    CaptureSink cb_sink;
+    CaptureSink cb_sink, ib_sink;
     long double ts[nblocks];
     {
         Pipeline pipeline(config, cbw, ibw, block_bytes);
@@ -646,6 +650,16 @@ void pipeline_outputs_match_operators()
             reader.data_stream().release();
         }
     }
+    {
+        DadaReadClient reader(ib_buffer.key(), log);
+        auto& h = reader.header_stream().next();
+        ib_sink.init(h);
+        reader.header_stream().release();
+        for (int i = 0; i < nblocks - 2; ++i) {
+            ib_sink(reader.data_stream().next());
+            reader.data_stream().release();
+        }
+    }
     EXPECT(cb_sink.header.find("SAMPLE_CLOCK_START 123456") != std::string::npos);
     EXPECT(cb_sink.blocks.size() == (std::size_t)(nblocks - 2));
     // reference result per block through the operator classes
@@ -653,7 +667,9 @@ void pipeline_outputs_match_operators()
     FBF_ERROR_CHECK(fbf_stream_create(&stream));
     SplitTranspose st(config);
     CoherentBeamformer cb(config);
+    IncoherentBeamformer ib(config);
     WeightsManager wm(config, stream);
+    EXPECT(ib_sink.blocks.size() == (std::size_t)(nblocks - 2));
     std::vector<float2> dh(simulator.delays(), simulator.delays() + nd);
     WeightsManager::DelayVectorType dd(dh);
     for (int i = 0; i < nblocks - 2; ++i) {
@@ -665,6 +681,11 @@ void pipeline_outputs_match_operators()
         std::vector<int8_t> expect = out.to_host();
         EXPECT(expect.size() == cb_sink.blocks[i].size());
         EXPECT(std::memcmp(expect.data(), cb_sink.blocks[i].data(), expect.size()) == 0);
+        IncoherentBeamformer::PowerVectorType ib_out;
+        ib.beamform(taftp, ib_out, stream);
+        std::vector<int8_t> ib_expect = ib_out.to_host();
+        EXPECT(ib_expect.size() == ib_sink.blocks[i].size());
+        EXPECT(std::memcmp(ib_expect.data(), ib_sink.blocks[i].data(), ib_expect.size()) == 0);
     }
     // the epochs advance by one block of samples
     EXPECT(std::fabs((double)(ts[1] - ts[0]) - 2.0 * 4096 * nhg * 256 / 1712e6) < 1e-9);
@@ -794,7 +815,8 @@ int main(int argc, char** argv)
         {"DelayManagerTester.test_updates", delay_manager_updates},
         {"DelayManagerTester.test_bad_keys", delay_manager_bad_keys},
         {"PipelineTester.simple_run_test", pipeline_simple_run},
-        {"Pipeline.outputs_match_operators", pipeline_outputs_match_operators},
+        {"Pipeline.outputs_match_operators_16ant", [] { pipeline_outputs_match_operators(16, 64); }},
+        {"Pipeline.outputs_match_operators_64ant_taftp", [] { pipeline_outputs_match_operators(64, 256); }},
         {"TransposeToDada.beams_match_oracle", tuse_transpose_to_dada},
         {"FengToDada.reorders_in_place", feng_to_dada_reorder},
     };

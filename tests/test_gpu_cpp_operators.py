@@ -19,7 +19,8 @@ CASES = [
     "CoherentBeamformer.error_paths_throw",
     "DelayManagerTester.test_updates",
     "PipelineTester.simple_run_test",
-    "Pipeline.outputs_match_operators",
+    "Pipeline.outputs_match_operators_16ant",
+    "Pipeline.outputs_match_operators_64ant_taftp",
     "TransposeToDada.beams_match_oracle",
     "FengToDada.reorders_in_place",
 ]

@@ -46,8 +46,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-channels", type=int, default=64, help="channels in the CPU sample (64 = whole block)")
     ap.add_argument("--variant", type=int, default=0, help="CB kernel variant (0 = auto)")
-    ap.add_argument("--path", choices=["taftp", "ftpa"], default="taftp",
-                    help="taftp: CB straight from TAFTP; ftpa: split-transpose pass, then CB on FTPA")
+    ap.add_argument("--path", choices=["fused", "taftp", "ftpa"], default="fused",
+                    help="fused: weights, then coherent and incoherent beams in one pass over TAFTP; "
+                         "taftp: weights + IB in one launch, then CB straight from TAFTP; "
+                         "ftpa: split-transpose pass (with IB and weights), then CB on FTPA")
     ap.add_argument("--separate-ib", action="store_true",
                     help="run the incoherent beam as its own kernel instead of inside the split-transpose pass")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 40/16/8 subarray leg")
@@ -115,7 +117,8 @@ def main():
     s = capi.Stream()
     d_taftps = [capi.DeviceBuffer.from_array(taftp if i == 0 else wl.synth_taftp(cfg, seed + 100 + i), s)
                 for i in range(NBLK)]
-    taftp_path = args.path == "taftp"
+    fused_path = args.path == "fused"
+    taftp_path = args.path in ("taftp", "fused")
     d_ftpa = None if taftp_path else capi.DeviceBuffer(cfg.ftpa_bytes(sub))
     d_delays = capi.DeviceBuffer.from_array(delays, s)
     d_w = capi.DeviceBuffer(cfg.weights_bytes(sub))
@@ -135,6 +138,8 @@ def main():
              and not args.separate_ib)
     if taftp_path and not cb.taftp_direct(cfg.total_nantennas, T):
         raise RuntimeError("the coherent-beam plan does not read this block straight from TAFTP")
+    if fused_path and not cb.ib_fused(ib, T):
+        raise RuntimeError("the plans do not form the incoherent beam inside the coherent kernel")
 
     def step(k, ev=None, weights_every=1):
         """One block on stream s.  A block that takes new weights generates
@@ -150,13 +155,18 @@ def main():
             if new_w:
                 wp.generate(d_delays, epoch, d_w, stream=s)
             ev[1].record(s)
-            if taftp_path:
+            if fused_path:
+                pass
+            elif taftp_path:
                 ib.beamform(d_taftp, d_ib, stream=s)
             elif fused:
                 capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
             else:
                 capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
             ev[2].record(s)
+        elif fused_path:
+            if new_w:
+                wp.generate(d_delays, epoch, d_w, stream=s)
         elif taftp_path:
             if new_w:
                 ib.beamform_weights(d_taftp, d_ib, wp, d_delays, epoch, d_w, stream=s)
@@ -171,7 +181,9 @@ def main():
                 capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
             else:
                 capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
-        if taftp_path:
+        if fused_path:
+            cb.ib_beamform_taftp(ib, d_taftp, d_w, d_cb, d_ib, stream=s)
+        elif taftp_path:
             cb.beamform_taftp(d_taftp, d_w, d_cb, nantennas_total=cfg.total_nantennas, stream=s)
         else:
             cb.beamform(d_ftpa, d_w, d_cb, stream=s)
@@ -236,7 +248,7 @@ def main():
     # The launch the timed blocks use for a block's weights: IB + weights
     # (FTPA path: split-transpose + IB + weights).
     stw_ms = None
-    if fused or taftp_path:
+    if fused or (taftp_path and not fused_path):
         fe = [capi.Event() for _ in range(R + 1)]
         fe[0].record(s)
         for r in range(R):
@@ -256,18 +268,23 @@ def main():
     pipeline_tops = ops / (ms_per_step * 1e-3) / 1e12
     samples_per_s = (F * T * wl.NPOL * cfg.total_nantennas) / (ms_per_step * 1e-3)
     ib_bytes = cfg.taftp_bytes + cfg.ib_out_bytes()
-    if taftp_path:
+    if fused_path:
+        st_bytes = 0
+        st_name = None
+    elif taftp_path:
         st_bytes = ib_bytes
         st_name = "incoherent_beamformer"
     else:
         st_bytes = 2 * cfg.ftpa_bytes(sub) + (cfg.ib_out_bytes() if fused else 0)
         st_name = "split_transpose_with_incoherent_beam" if fused else "split_transpose"
-    kernel_ms = {"weights": round(float(w_ms), 4), st_name: round(float(st_ms), 4),
-                 "coherent_beamformer": round(float(cb_ms), 4)}
+    kernel_ms = {"weights": round(float(w_ms), 4),
+                 ("coherent_and_incoherent_beams" if fused_path else "coherent_beamformer"): round(float(cb_ms), 4)}
+    if st_name:
+        kernel_ms[st_name] = round(float(st_ms), 4)
     if stw_ms is not None:
         kernel_ms["ib_weights_one_launch" if taftp_path else "split_transpose_ib_weights_one_launch"] = \
             round(float(stw_ms), 4)
-    kernel_gbs = {st_name: round(st_bytes / (st_ms * 1e-3) / 1e9, 1)}
+    kernel_gbs = {st_name: round(st_bytes / (st_ms * 1e-3) / 1e9, 1)} if st_name else {}
     if not fused and not taftp_path:
         kernel_ms["incoherent_beamformer"] = round(float(ib_ms), 4)
         kernel_gbs["incoherent_beamformer"] = round(ib_bytes / (ib_ms * 1e-3) / 1e9, 1)
@@ -294,7 +311,9 @@ def main():
             "cb_tscrunch": sub.tscrunch, "cb_fscrunch": sub.fscrunch,
             "ib_tscrunch": cfg.ib_tscrunch, "ib_fscrunch": cfg.ib_fscrunch,
             "block_ms": round(block_s * 1e3, 3),
-            "stages": ("weights (every block) and the incoherent beam in one launch, then the coherent beams "
+            "stages": ("weights (every block), then the coherent and incoherent beams in one pass over TAFTP "
+                       "(the coherent kernel leaves per-sample IB partials; no split-transpose pass)" if fused_path
+                       else "weights (every block) and the incoherent beam in one launch, then the coherent beams "
                        "straight from TAFTP (no split-transpose pass)" if taftp_path
                        else "weights (every block), split-transpose and IB in one launch, then CB" if fused
                        else "weights (every block) + split-transpose + CB + IB"),
@@ -327,7 +346,9 @@ def main():
                               "on this workload, scaled by a plain copy's readings, from "
                               "profiles/cb_64ant_64ch_512beams_hbm_traffic.txt (not measured in this run)",
             "ops_per_launch": ops,
-            "window": ("HIP events around fbf_cb_beamform_taftp on its stream: cb_tr_k (staging the TAFTP "
+            "window": ("HIP events around fbf_cb_ib_beamform_taftp on its stream: cb_tr_k (staging the TAFTP "
+                       "block itself, with the IB partials) plus the tbtf_k reorder and ib_finish_k" if fused_path
+                       else "HIP events around fbf_cb_beamform_taftp on its stream: cb_tr_k (staging the TAFTP "
                        "block itself) plus the tbtf_k reorder" if taftp_path else
                        "HIP events around fbf_cb_beamform on its stream: cb_tr_k plus the tbtf_k reorder"),
         },
@@ -336,7 +357,8 @@ def main():
     gpu_weights = d_w.download()      # the last block's weights, for the CPU baseline's check
     # the epoch of the last weights written to d_w (the fused launches above
     # when the block uses them, else the per-kernel pass)
-    gpu_weights_epoch = cfg.block_seconds * ((R - 1) if (fused or taftp_path) else (args.warmup + 2 * K + R - 1))
+    gpu_weights_epoch = cfg.block_seconds * ((R - 1) if (fused or (taftp_path and not fused_path))
+                                             else (args.warmup + 2 * K + R - 1))
     wp.close()
     cb.close()
     ib.close()

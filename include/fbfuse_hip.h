@@ -246,6 +246,24 @@ int fbf_ib_beamform(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes,
 /* Exact power, double [T/ts][F/fs]. */
 int fbf_ib_power(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes,
                  double* power, size_t power_bytes, fbf_stream_t stream);
+/* Coherent and incoherent beams of one TAFTP block (all the coherent plan's
+ * antennas; no subset) in one pass over it: outputs equal those of
+ * fbf_cb_beamform_taftp(plan, ..., nantennas_total = plan nantennas,
+ * start_antenna = 0, ...) and fbf_ib_beamform(ib_plan, ...).  Replaces the
+ * reference Pipeline's three operator calls (Pipeline.cu:166-170).  Where
+ * fbf_cb_ib_fused says so, the coherent kernel also leaves each sample's
+ * power over the antennas and pols (from the operands it already holds)
+ * in a buffer the coherent plan owns, and a small kernel scrunches and
+ * quantises them; otherwise the two beamformers run one after the other. */
+int fbf_cb_ib_beamform_taftp(fbf_cb_plan* plan, fbf_ib_plan* ib_plan, const void* taftp, size_t taftp_bytes,
+                             int nsamples_per_heap, const void* fba_weights, size_t weights_bytes,
+                             void* tbtf, size_t tbtf_bytes, void* tf, size_t tf_bytes, fbf_stream_t stream);
+/* *fused = 1 when fbf_cb_ib_beamform_taftp forms the incoherent beam inside
+ * the coherent kernel: the direct TAFTP path with 128 beams per wave, and
+ * an incoherent-beam plan of the same antennas, channels and heap size.  No
+ * counterpart in the reference. */
+int fbf_cb_ib_fused(const fbf_cb_plan* plan, const fbf_ib_plan* ib_plan, int nsamples_per_heap,
+                    long long nsamples, int* fused);
 
 /* Split-transpose of a full-antenna block (all antennas, no subset) and the
  * incoherent beam of the same block in one pass over the input: the FTPA

@@ -113,6 +113,8 @@ _SIGS = {
     "fbf_cb_beamform_subarray": [vp, vp, sz, i32, i32, vp, sz, vp, sz, vp],
     "fbf_cb_beamform_taftp": [vp, vp, sz, i32, i32, i32, vp, sz, vp, sz, vp],
     "fbf_cb_taftp_direct": [vp, i32, i32, C.c_longlong, C.POINTER(C.c_int)],
+    "fbf_cb_ib_beamform_taftp": [vp, vp, vp, sz, i32, vp, sz, vp, sz, vp, sz, vp],
+    "fbf_cb_ib_fused": [vp, vp, i32, C.c_longlong, C.POINTER(C.c_int)],
     "fbf_cb_plan_set_chunk": [vp, i32],
     "fbf_cb_autotune": [vp, C.c_longlong, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)],
     "fbf_batch_create": [i32, i32, i32, i32, i32, C.POINTER(SubarrayDesc), C.POINTER(C.c_void_p),
@@ -443,6 +445,21 @@ class CbPlan:
         call("fbf_cb_beamform_taftp", self.handle, taftp.ptr, taftp.nbytes, int(nantennas_total),
              int(start_antenna), int(nsamples_per_heap), weights.ptr, weights.nbytes, out.ptr, out.nbytes,
              _s(stream))
+
+    def ib_beamform_taftp(self, ib_plan, taftp, weights, out, tf, nsamples_per_heap=256, stream=None):
+        """Coherent beams (out) and the incoherent beam of ib_plan (tf) of a
+        TAFTP block of this plan's antennas, in one pass where the plans
+        allow it (ib_fused)."""
+        call("fbf_cb_ib_beamform_taftp", self.handle, ib_plan.handle, taftp.ptr, taftp.nbytes,
+             int(nsamples_per_heap), weights.ptr, weights.nbytes, out.ptr, out.nbytes, tf.ptr, tf.nbytes,
+             _s(stream))
+
+    def ib_fused(self, ib_plan, nsamples, nsamples_per_heap=256):
+        """Whether ib_beamform_taftp forms the incoherent beam inside the
+        coherent kernel."""
+        f = C.c_int()
+        call("fbf_cb_ib_fused", self.handle, ib_plan.handle, int(nsamples_per_heap), int(nsamples), C.byref(f))
+        return bool(f.value)
 
     def taftp_direct(self, nantennas_total, nsamples, nsamples_per_heap=256):
         """Whether beamform_taftp reads such a block without an FTPA copy."""

@@ -946,7 +946,37 @@ struct CbTrArgs {
     // (16 samples) per heap.
     size_t hstride;
     int lgtph;
+    // Incoherent-beam partials (cb_tr_k with IBF): [F][ntiles][2 halves][32
+    // columns] int32; sample j of a tile has the power over the block's
+    // antennas and both pols split over columns 2j, 2j + 1 (its pols) of
+    // both halves (ib_finish_k adds the four).
+    int* ibp;
 };
+
+// The incoherent-beam partials of tiles [t0, t1) of channel f straight from
+// the TAFTP block, for the work-groups on the exact path (which leave the
+// tile loop): per sample the power over all antennas and both pols in
+// column 2j of half 0, zeros in its other three places.
+__device__ __forceinline__ void cb_exact_ib_partials(const CbArgs& p, int* ibp, int ntiles, int f, int t0, int t1,
+                                                  int tid, int nt)
+{
+    const size_t astep = (size_t)p.F * p.S * 4;
+    for (int i = tid; i < (t1 - t0) * 16; i += nt) {
+        const int tile = t0 + i / 16, j = i % 16;
+        const long long t = (long long)tile * 16 + j;
+        const int8_t* v = p.ftpa + (((size_t)(t / p.S) * p.Atot + p.a0) * p.F + f) * p.S * 4 + (size_t)(t % p.S) * 4;
+        int sum = 0;
+        for (int a = 0; a < p.A; ++a) {
+            const int w = *reinterpret_cast<const int*>(v + a * astep);
+            sum = __builtin_amdgcn_sdot4(w, w, sum, false);
+        }
+        int* dst = ibp + ((size_t)f * ntiles + tile) * 64 + 2 * j;
+        dst[0] = sum;
+        dst[1] = 0;
+        dst[32] = 0;
+        dst[33] = 0;
+    }
+}
 
 // The 16-byte A operands of one step's K slices (8 antennas x char2 of one
 // column each) from an LDS tile laid out [antenna][32 columns] int16, 64
@@ -1020,6 +1050,28 @@ __device__ __forceinline__ void tr_operands(const int8_t* src, v4i (&a)[KSTEPS])
     }
 #pragma unroll
     for (int s = 0; s < KSTEPS; ++s) a[s] = v4i{r[2 * s].x, r[2 * s].y, r[2 * s + 1].x, r[2 * s + 1].y};
+}
+
+// A tile's incoherent-beam partials from the operands tr_operands gave this
+// wave (every K slice): per lane the power of its column over the 8 * KSTEPS
+// antennas of its row half, each word (two antennas' re, im) squared with
+// v_dot4, antenna groups of 8 past A left out (they re-read antenna 0); the
+// 64 lanes store their sums as the tile's [2 halves][32 columns] record.
+template <int KSTEPS>
+__device__ __forceinline__ void ib_tile(const v4i (&a)[KSTEPS], int* ibp, int ntiles, int A, int f, int tile,
+                                        int lane)
+{
+    const int h = lane >> 5;
+    int v = 0;
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) {
+        int t = __builtin_amdgcn_sdot4(a[s].x, a[s].x, 0, false);
+        t = __builtin_amdgcn_sdot4(a[s].y, a[s].y, t, false);
+        t = __builtin_amdgcn_sdot4(a[s].z, a[s].z, t, false);
+        t = __builtin_amdgcn_sdot4(a[s].w, a[s].w, t, false);
+        v += 8 * (h * KSTEPS + s) < A ? t : 0;
+    }
+    if (tile < ntiles) ibp[((size_t)f * ntiles + tile) * 64 + lane] = v;
 }
 
 // a + b' where lanes below 32 keep a and take b' = (a of lane + 32), and
@@ -1189,7 +1241,9 @@ struct TrRed {
 // per pass held in registers; FS: fscrunch; NTS: time slots per work-group;
 // TA: the voltages are a TAFTP block (CbArgs::S > 0), staged straight into
 // [antenna][column] tiles and read with transposed LDS loads (tr_operands),
-// else FTPA columns.
+// else FTPA columns; IBF (with TA, FS = 1, all antennas of the block): the
+// work-groups of beam group 0 also leave the incoherent beam's per-sample
+// partials (CbTrArgs::ibp) from the same operands.
 //
 // A work-group has 4 * NTS waves: NTS time slots (contiguous runs of passes
 // of the same channel group and beams) x 4 waves ((4 / FS) beam groups x FS
@@ -1198,7 +1252,7 @@ struct TrRed {
 // per-step barrier keeps them in lockstep: waves of separate work-groups
 // sharing a SIMD progress at different rates (the issue arbiter prefers
 // older waves), which leaves SIMDs with one wave at the end.
-template <int RT, int KSTEPS, int SPT, int NSLOT, int FS, int NTS, bool TA>
+template <int RT, int KSTEPS, int SPT, int NSLOT, int FS, int NTS, bool TA, bool IBF>
 __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 {
     using St = TrStage<FS, KSTEPS, NTS>;
@@ -1210,6 +1264,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     static_assert(RT == 2 || RT == 4 || RT == 8, "RT");
     static_assert(NSLOT % SPT == 0 && (RT != 2 || NSLOT % 2 == 0), "slots");
     static_assert(St::M <= 2, "at most two staging loads per wave per set");
+    static_assert(!IBF || (TA && FS == 1 && KSTEPS <= 4 && NTS == 2), "incoherent-beam partials");
     extern __shared__ __align__(16) int8_t lds[];
     const CbArgs& p = ta.p;
 
@@ -1376,6 +1431,11 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
                                         (long long)tgroup * NTS * slot_tiles * SPT / ta.tps,
                                         (long long)(tend < ta.ntiles ? tend : ta.ntiles) * SPT / ta.tps, tid,
                                         (int)blockDim.x);
+            if constexpr (IBF) {
+                if (bgroup == 0)
+                    cb_exact_ib_partials(p, ta.ibp, ta.ntiles, fo, tgroup * NTS * slot_tiles,
+                                         tend < ta.ntiles ? tend : ta.ntiles, tid, (int)blockDim.x);
+            }
             return;
         }
     }
@@ -1440,7 +1500,16 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
         // Passes past the end of the channel (the last slots of a one-round
         // grid) still take part in the staging and the barriers but leave
         // the MFMA pipe to the other waves of the SIMD.
-        const bool live = wave_active && tile0 + pass * ta.ntp < ta.ntiles;
+        const bool tiles_live = tile0 + pass * ta.ntp < ta.ntiles;
+        const bool live = wave_active && tiles_live;
+        // Incoherent-beam partials: in beam group 0, one wave of each time
+        // slot per step (whether it holds beams or not) forms the tile's.
+        // The role rotates with the step and the time slot, so that every
+        // SIMD (which holds the waves of one beam-group index) takes one tile
+        // per two-step barrier interval, and a wave issues one partials
+        // store per four steps (stores count against the staging loads'
+        // vmcnt wait).
+        const bool ib_live = IBF && bgroup == 0 && tiles_live;
 
         // Tiles of the pass: NSLOT/SPT groups of tps tiles (tps = 1 unless
         // a window spans several tiles).
@@ -1460,7 +1529,8 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
                         is = is + 1 == NB ? 0 : is + 1;
                     }
                 }
-                if (live) {
+                const bool ib_now = ib_live && ((step + 2 * tsl) & 3) == (wv & 3);
+                if (live || ib_now) {
                     const int8_t* src = lds + rs * St::SET;
                     v4i a[KSTEPS];
                     if constexpr (TA) {
@@ -1469,7 +1539,11 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 #pragma unroll
                         for (int s = 0; s < KSTEPS; ++s) a[s] = *reinterpret_cast<const v4i*>(src + aoff[s]);
                     }
-                    detect_step<RT, KSTEPS, SPT, NSLOT>(a, w, pw, g);
+                    if (live) detect_step<RT, KSTEPS, SPT, NSLOT>(a, w, pw, g);
+                    if constexpr (IBF) {
+                        if (ib_now) ib_tile<KSTEPS>(a, ta.ibp, ta.ntiles, p.A, fo, tile0 + pass * ta.ntp + g * ta.tps + x,
+                                                    lane);
+                    }
                 }
                 rs = rs + 1 == NB ? 0 : rs + 1;
                 ++step;
@@ -1580,7 +1654,7 @@ inline int device_cu_count()
 // contiguous run of passes, so the per-work-group start-up (weights, the
 // first staged tiles) is paid once per resident slot and all work-groups end
 // together.
-template <int RT, int KSTEPS, int SPT, int NSLOT, int FS, bool TA>
+template <int RT, int KSTEPS, int SPT, int NSLOT, int FS, bool TA, bool IBF = false>
 hipError_t launch_tr_kt(CbTrArgs a, int passes, hipStream_t s)
 {
     constexpr int NTS = tr_nts(RT, NSLOT);
@@ -1588,7 +1662,7 @@ hipError_t launch_tr_kt(CbTrArgs a, int passes, hipStream_t s)
     static int occ = 0;
     if (occ == 0) {
         int o = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, cb_tr_k<RT, KSTEPS, SPT, NSLOT, FS, NTS, TA>, 256 * NTS,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, cb_tr_k<RT, KSTEPS, SPT, NSLOT, FS, NTS, TA, IBF>, 256 * NTS,
                                                          shm) != hipSuccess ||
             o < 1)
             o = 1;
@@ -1606,18 +1680,25 @@ hipError_t launch_tr_kt(CbTrArgs a, int passes, hipStream_t s)
     a.n_tgroups = (int)((passes + (long long)a.npasses * NTS - 1) / ((long long)a.npasses * NTS));
     const long long nblocks = units * a.n_tgroups;
     if (nblocks <= 0 || nblocks >= (1ll << 31)) return hipErrorInvalidConfiguration;
-    cb_tr_k<RT, KSTEPS, SPT, NSLOT, FS, NTS, TA><<<dim3((unsigned)nblocks), 256 * NTS, shm, s>>>(a);
+    cb_tr_k<RT, KSTEPS, SPT, NSLOT, FS, NTS, TA, IBF><<<dim3((unsigned)nblocks), 256 * NTS, shm, s>>>(a);
     return hipGetLastError();
 }
 
 // TAFTP staging (a.p.S > 0) is built for 64 and 128 beams per wave and
 // fscrunch 1 only (cb_taftp_direct): with 64 beams per wave no fscrunch above
-// 1 fits the kernel's LDS (cb_tr_supported).
+// 1 fits the kernel's LDS (cb_tr_supported).  The incoherent-beam partials
+// (a.ibp) are built for 128 beams per wave (cb_ib_fused).
 template <int RT, int KSTEPS, int SPT, int NSLOT, int FS>
 hipError_t launch_tr_k(CbTrArgs a, int passes, hipStream_t s)
 {
     if (a.p.S > 0) {
-        if constexpr (RT != 2 && FS == 1) return launch_tr_kt<RT, KSTEPS, SPT, NSLOT, FS, true>(a, passes, s);
+        if constexpr (RT != 2 && FS == 1) {
+            if (a.ibp) {
+                if constexpr (RT == 8) return launch_tr_kt<RT, KSTEPS, SPT, NSLOT, FS, true, true>(a, passes, s);
+                return hipErrorInvalidValue;
+            }
+            return launch_tr_kt<RT, KSTEPS, SPT, NSLOT, FS, true>(a, passes, s);
+        }
         return hipErrorInvalidValue;
     }
     return launch_tr_kt<RT, KSTEPS, SPT, NSLOT, FS, false>(a, passes, s);
@@ -2269,6 +2350,8 @@ struct fbf_cb_plan {
     size_t staging_bytes;
     void* ftpa;       // FTPA of fbf_cb_beamform_taftp's split-transpose path
     size_t ftpa_bytes;
+    void* ibp;        // incoherent-beam partials of fbf_cb_ib_beamform_taftp
+    size_t ibp_bytes;
 };
 
 static long long* cb_thr(const fbf_cb_plan* plan)
@@ -2342,7 +2425,7 @@ extern "C" int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets,
     const int Fo = desc->nchans / desc->fscrunch;
     int ohs = desc->nsamples_per_heap_out ? desc->nsamples_per_heap_out : 8192 / Fo;
     FBF_REQUIRE(ohs > 0, FBF_ERR_ARG, "fbf_cb_plan_create: output heap holds no samples");
-    auto* p = new fbf_cb_plan{*desc, ohs, 0, 0, nullptr, nullptr, 0, nullptr, 0};
+    auto* p = new fbf_cb_plan{*desc, ohs, 0, 0, nullptr, nullptr, 0, nullptr, 0, nullptr, 0};
     hipError_t e = hipMalloc(&p->d_levels, sizeof(float) * 2 * Fo + sizeof(long long) * kThr * Fo);
     if (e != hipSuccess) {
         delete p;
@@ -2393,6 +2476,10 @@ extern "C" int fbf_cb_plan_destroy(fbf_cb_plan* plan)
     }
     if (plan->ftpa) {
         const hipError_t e2 = hipFree(plan->ftpa);
+        if (e == hipSuccess) e = e2;
+    }
+    if (plan->ibp) {
+        const hipError_t e2 = hipFree(plan->ibp);
         if (e == hipSuccess) e = e2;
     }
     delete plan;
@@ -2503,6 +2590,7 @@ extern "C" int fbf_cb_output_bytes(const fbf_cb_plan* plan, size_t ftpa_bytes, s
 struct TaftpView {
     int S, Atot, a0;
     int variant;   // the kernel variant to pick from (cb_taftp_variant)
+    int* ibp;      // incoherent-beam partials to leave (cb_ib_fused), or null
 };
 
 // Validates and launches one coherent-beam kernel over T samples of voltage
@@ -2622,6 +2710,7 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
             a.a0 = tv->a0;
             ta.hstride = (size_t)4 * tv->Atot * d.nchans * tv->S;
             ta.lgtph = __builtin_ctz((unsigned)(tv->S / 16));
+            ta.ibp = tv->ibp;
         }
         const int rt = var.rows;
         const int W = 2 * d.tscrunch;
@@ -2795,7 +2884,7 @@ extern "C" int fbf_cb_beamform_taftp(fbf_cb_plan* plan, const void* taftp, size_
     FBF_REQUIRE((reinterpret_cast<uintptr_t>(taftp) & 15) == 0, FBF_ERR_ARG,
                 "fbf_cb_beamform_taftp: buffers must be 16-byte aligned");
     if (T == 0 || cb_taftp_direct(plan, T, nantennas_total, nsamples_per_heap)) {
-        const TaftpView tv{nsamples_per_heap, nantennas_total, start_antenna, cb_taftp_variant(plan, T)};
+        const TaftpView tv{nsamples_per_heap, nantennas_total, start_antenna, cb_taftp_variant(plan, T), nullptr};
         return cb_launch(plan, MODE_INT8, taftp, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf,
                          tbtf_bytes, stream, nullptr, 0, T ? &tv : nullptr);
     }
@@ -2819,6 +2908,71 @@ extern "C" int fbf_cb_beamform_taftp(fbf_cb_plan* plan, const void* taftp, size_
     if (st) return st;
     return cb_launch(plan, MODE_INT8, ftpa, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf, tbtf_bytes,
                      stream);
+}
+
+// Whether fbf_cb_ib_beamform_taftp forms the incoherent beam inside the
+// coherent beamformer: the direct TAFTP path with 128 beams per wave over
+// all the block's antennas, and an incoherent-beam plan of the same block
+// (antennas, channels, heap size).
+static bool cb_ib_fused(const fbf_cb_plan* plan, const IbFuseParams& ibp, long long T, int S)
+{
+    const fbf_cb_desc& d = plan->d;
+    const CbVariant var = cb_pick_variant(d, cb_taftp_variant(plan, T), MODE_INT8, T);
+    return cb_taftp_direct(plan, T, d.nantennas, S) && var.rows == 8 && ibp.nantennas == d.nantennas &&
+           ibp.nchans == d.nchans && ibp.nsamples_per_heap == S;
+}
+
+extern "C" int fbf_cb_ib_beamform_taftp(fbf_cb_plan* plan, fbf_ib_plan* ib_plan, const void* taftp,
+                                        size_t taftp_bytes, int nsamples_per_heap, const void* fba_weights,
+                                        size_t weights_bytes, void* tbtf, size_t tbtf_bytes, void* tf,
+                                        size_t tf_bytes, fbf_stream_t stream)
+{
+    FBF_REQUIRE(plan && ib_plan, FBF_ERR_ARG, "fbf_cb_ib_beamform_taftp: null plan");
+    FBF_REQUIRE(taftp && fba_weights && tbtf && tf, FBF_ERR_ARG, "fbf_cb_ib_beamform_taftp: null buffer");
+    IbFuseParams ibp;
+    int st = ib_fuse_params(ib_plan, &ibp);
+    if (st) return st;
+    const fbf_cb_desc& d = plan->d;
+    FBF_REQUIRE(nsamples_per_heap > 0, FBF_ERR_ARG, "fbf_cb_ib_beamform_taftp: nsamples_per_heap must be positive");
+    const size_t hg_bytes = (size_t)4 * d.nantennas * d.nchans * nsamples_per_heap;
+    FBF_REQUIRE(taftp_bytes % hg_bytes == 0, FBF_ERR_SIZE,
+                "fbf_cb_ib_beamform_taftp: input is not a whole number of heap groups of the plan's antennas");
+    const long long T = (long long)(taftp_bytes / hg_bytes) * nsamples_per_heap;
+    if (T == 0 || !cb_ib_fused(plan, ibp, T, nsamples_per_heap)) {
+        // The two beamformers one after the other.
+        st = fbf_ib_beamform(ib_plan, taftp, taftp_bytes, tf, tf_bytes, stream);
+        if (st) return st;
+        return fbf_cb_beamform_taftp(plan, taftp, taftp_bytes, d.nantennas, 0, nsamples_per_heap, fba_weights,
+                                     weights_bytes, tbtf, tbtf_bytes, stream);
+    }
+    size_t tf_expect = 0;
+    st = fbf_ib_output_bytes(ib_plan, taftp_bytes, &tf_expect);
+    if (st) return st;
+    FBF_REQUIRE(tf_bytes == tf_expect, FBF_ERR_SIZE,
+                "fbf_cb_ib_beamform_taftp: incoherent-beam output size " + std::to_string(tf_bytes) +
+                    " != expected " + std::to_string(tf_expect));
+    FBF_REQUIRE((reinterpret_cast<uintptr_t>(taftp) & 15) == 0, FBF_ERR_ARG,
+                "fbf_cb_ib_beamform_taftp: buffers must be 16-byte aligned");
+    void* part = nullptr;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    st = cb_grow(&plan->ibp, &plan->ibp_bytes, (size_t)d.nchans * T * 4 * sizeof(int), s, &part);
+    if (st) return st;
+    const TaftpView tv{nsamples_per_heap, d.nantennas, 0, cb_taftp_variant(plan, T), static_cast<int*>(part)};
+    st = cb_launch(plan, MODE_INT8, taftp, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf, tbtf_bytes,
+                   stream, nullptr, 0, &tv);
+    if (st) return st;
+    return ib_finish_launch(ib_plan, static_cast<const int*>(part), T, tf, tf_bytes, s);
+}
+
+extern "C" int fbf_cb_ib_fused(const fbf_cb_plan* plan, const fbf_ib_plan* ib_plan, int nsamples_per_heap,
+                               long long nsamples, int* fused)
+{
+    FBF_REQUIRE(plan && ib_plan && fused, FBF_ERR_ARG, "fbf_cb_ib_fused: null argument");
+    IbFuseParams ibp;
+    const int st = ib_fuse_params(ib_plan, &ibp);
+    if (st) return st;
+    *fused = nsamples > 0 && nsamples % plan->d.tscrunch == 0 && cb_ib_fused(plan, ibp, nsamples, nsamples_per_heap);
+    return FBF_OK;
 }
 
 extern "C" int fbf_cb_taftp_direct(const fbf_cb_plan* plan, int nantennas_total, int nsamples_per_heap,

@@ -189,3 +189,65 @@ def test_ib_beamform_weights_rejects_channel_mismatch(gpu):
     d_w = capi.DeviceBuffer((F // 2) * B * A * 2)
     with pytest.raises(capi.FbfError):
         ib.beamform_weights(d_t, d_ib, wp, d_d, 0.0, d_w)
+
+
+def _check_cb_ib(capi, oracle, A, F, nhg, B, ts, ib_ts, ib_fs, S, variant, fused, seed, weights=None):
+    """fbf_cb_ib_beamform_taftp against the oracle: the coherent beams as in
+    _check_taftp, the incoherent beam as the oracle's ib_power + ib_tf over
+    all antennas (IncoherentBeamformer.cu:10-69)."""
+    T = nhg * S
+    taftp = gaussian_int8(seed, nhg * A * F * S * 4, 16.0)
+    w = unit_weights(seed + 1, F * B * A) if weights is None else weights
+    off, sc = _levels(oracle, A, ts, 1, F)
+    _, _, ibo, ibs = oracle.power_levels(16.0, 24.0, A, ts, 1, A, ib_ts, ib_fs)
+    Fo = F // ib_fs
+    ioff, isc = np.full(Fo, ibo, np.float32), np.full(Fo, ibs, np.float32)
+    plan = capi.CbPlan(A, B, F, ts, 1, off, sc, nsamples_per_heap_out=T // ts)
+    if variant:
+        plan.set_variant(variant)
+    ib = capi.IbPlan(A, F, ib_ts, ib_fs, ioff, isc, nsamples_per_heap=S)
+    assert plan.ib_fused(ib, T, S) == fused
+    ftpa = oracle.split_transpose(taftp, nhg, A, A, 0, F, nsamples_per_heap=S)
+    ref = oracle.cb_tbtf(oracle.cb_power(ftpa, w, F, T, A, B, ts, 1), off, sc, T // ts)
+    ib_ref = oracle.ib_tf(oracle.ib_power(taftp, F, ib_ts, ib_fs, nhg, A, nsamples_per_heap=S), ioff, isc)
+    d_t, d_w = _dev(capi, taftp), _dev(capi, w)
+    d_o = capi.DeviceBuffer(F * (T // ts) * B)
+    d_tf = capi.DeviceBuffer(ib.output_bytes(d_t.nbytes))
+    for _ in range(2):   # a second call finds the partials buffer in use
+        d_o.fill(0x5A)
+        d_tf.fill(0x5A)
+        plan.ib_beamform_taftp(ib, d_t, d_w, d_o, d_tf, nsamples_per_heap=S)
+        np.testing.assert_array_equal(d_o.download(), ref)
+        np.testing.assert_array_equal(d_tf.download(), ib_ref.reshape(-1))
+
+
+@pytest.mark.parametrize("A,F,nhg,B,ts,ib_ts,ib_fs,S,variant", [
+    (64, 8, 2, 512, 16, 16, 1, 256, 0),     # the bench geometry (channels and heaps cut)
+    (64, 4, 2, 256, 4, 1, 1, 256, 0),       # IB per sample, CB tscrunch 4
+    (64, 4, 3, 512, 8, 256, 2, 256, 0),     # IB windows of a whole heap, fscrunch 2
+    (64, 8, 2, 1024, 32, 8, 4, 256, 0),     # two beam groups (only the first leaves partials), IB fscrunch 4
+    (40, 8, 2, 200, 16, 16, 1, 256, 11),    # antennas 40..47 padded; waves without beams
+    (48, 4, 2, 128, 64, 32, 1, 256, 11),    # three K slices: the fourth wave's slot holds zeros
+    (16, 4, 4, 128, 16, 16, 1, 256, 11),    # one K slice
+])
+def test_cb_ib_fused_bit_exact(gpu, oracle, A, F, nhg, B, ts, ib_ts, ib_fs, S, variant):
+    _check_cb_ib(gpu, oracle, A, F, nhg, B, ts, ib_ts, ib_fs, S, variant, True, 2500 + A + B + ib_ts)
+
+
+@pytest.mark.parametrize("A,F,nhg,B,ts,ib_ts,ib_fs,S,variant", [
+    (64, 4, 2, 256, 16, 16, 1, 256, 9),     # 64 beams per wave: the two beamformers in turn
+    (40, 8, 2, 96, 16, 16, 2, 256, 0),      # the automatic pick at 40 antennas
+])
+def test_cb_ib_unfused_bit_exact(gpu, oracle, A, F, nhg, B, ts, ib_ts, ib_fs, S, variant):
+    _check_cb_ib(gpu, oracle, A, F, nhg, B, ts, ib_ts, ib_fs, S, variant, False, 2600 + A + B + ib_ts)
+
+
+def test_cb_ib_fused_minus128_weights_exact(gpu, oracle):
+    """Work-groups with a weight component of -128 take the exact path and
+    leave the incoherent-beam partials from the TAFTP block themselves."""
+    A, F, nhg, B = 64, 4, 2, 512
+    w = unit_weights(2701, F * B * A).reshape(F, B, A, 2).copy()
+    w[0, 3, 5, 1] = -128
+    w[F - 1, B - 7, 0, 1] = -128
+    w[2, 100, :, :] = -128
+    _check_cb_ib(gpu, oracle, A, F, nhg, B, 16, 16, 1, 256, 0, True, 2700, weights=w.reshape(-1))

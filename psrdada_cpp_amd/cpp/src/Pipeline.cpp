@@ -135,7 +135,23 @@ void Pipeline::init(RawBytes& header)
 void Pipeline::process(VoltageVectorType const& taftp, PowerVectorType& tbtf, PowerVectorType& tf)
 {
     auto const& delays = _delay_manager->delays();
-    int direct = 0;
+    int one_pass = 0, direct = 0;
+    if (_config.cb_nantennas() == _config.total_nantennas() && _config.cb_antenna_offset() == 0)
+        FBF_ERROR_CHECK(fbf_cb_ib_fused(_cb_plan, _ib_plan, (int)_config.nsamples_per_heap(),
+                                        (long long)_nsamples_per_dada_block, &one_pass));
+    if (one_pass) {
+        // The weights, then both beams in one pass over the TAFTP block (the
+        // coherent kernel leaves the incoherent beam's partials): the same
+        // outputs as the reference's order (Pipeline.cu:164-170).
+        FBF_ERROR_CHECK(fbf_weights_generate(_weights_plan, delays.data(), delays.bytes(),
+                                             (double)_unix_timestamp, _weights.data(), _weights.bytes(), nullptr,
+                                             _processing_stream));
+        FBF_ERROR_CHECK(fbf_cb_ib_beamform_taftp(_cb_plan, _ib_plan, taftp.data(), taftp.bytes(),
+                                                 (int)_config.nsamples_per_heap(), _weights.data(), _weights.bytes(),
+                                                 tbtf.data(), tbtf.bytes(), tf.data(), tf.bytes(),
+                                                 _processing_stream));
+        return;
+    }
     FBF_ERROR_CHECK(fbf_cb_taftp_direct(_cb_plan, (int)_config.total_nantennas(), (int)_config.nsamples_per_heap(),
                                         (long long)_nsamples_per_dada_block, &direct));
     if (direct) {

@@ -569,8 +569,9 @@ void pipeline_simple_run()
 // from shared memory and weights at each block's unix time: coherent beams
 // through SplitTranspose + CoherentBeamformer, the incoherent beam through
 // IncoherentBeamformer.  At 16 antennas the pipeline split-transposes (the
-// reference's order); at 64 it forms the coherent beams straight from the
-// TAFTP block with the weights in the incoherent beam's launch.
+// reference's order); at 64 it forms both beams in one pass over the TAFTP
+// block; for 64 antennas of a 128-antenna block it forms the coherent beams
+// straight from the TAFTP block, the weights in the incoherent beam's launch.
 struct CaptureSink {
     std::vector<std::vector<int8_t>> blocks;
     std::string header;
@@ -582,12 +583,13 @@ struct CaptureSink {
     }
 };
 
-void pipeline_outputs_match_operators(int A, int B)
+void pipeline_outputs_match_operators(int Atot, int A, int a0, int B)
 {
     PipelineConfig config;
-    config.total_nantennas(A);
+    config.total_nantennas(Atot);
     config.nchans(16);
     config.cb_nantennas(A);
+    config.cb_antenna_offset(a0);
     config.cb_nbeams(B);
     config.cb_fscrunch(1);
     config.cb_tscrunch(16);
@@ -608,7 +610,7 @@ void pipeline_outputs_match_operators(int A, int B)
     simulator.update_delays();
 
     const int nhg = 8;
-    const std::size_t block_bytes = (std::size_t)nhg * A * 16 * 256 * 2 * 2;
+    const std::size_t block_bytes = (std::size_t)nhg * Atot * 16 * 256 * 2 * 2;
     const std::size_t cb_bytes = (std::size_t)B * (nhg * 256 / 16) * 16;
     const std::size_t ib_bytes = (nhg * 256 / 16) * 16;
     DadaDB cb_buffer(16, cb_bytes, 4, 4096), ib_buffer(16, ib_bytes, 4, 4096);
@@ -815,8 +817,10 @@ int main(int argc, char** argv)
         {"DelayManagerTester.test_updates", delay_manager_updates},
         {"DelayManagerTester.test_bad_keys", delay_manager_bad_keys},
         {"PipelineTester.simple_run_test", pipeline_simple_run},
-        {"Pipeline.outputs_match_operators_16ant", [] { pipeline_outputs_match_operators(16, 64); }},
-        {"Pipeline.outputs_match_operators_64ant_taftp", [] { pipeline_outputs_match_operators(64, 256); }},
+        {"Pipeline.outputs_match_operators_16ant", [] { pipeline_outputs_match_operators(16, 16, 0, 64); }},
+        {"Pipeline.outputs_match_operators_64ant_fused", [] { pipeline_outputs_match_operators(64, 64, 0, 256); }},
+        {"Pipeline.outputs_match_operators_64of128_taftp",
+         [] { pipeline_outputs_match_operators(128, 64, 64, 256); }},
         {"TransposeToDada.beams_match_oracle", tuse_transpose_to_dada},
         {"FengToDada.reorders_in_place", feng_to_dada_reorder},
     };

@@ -20,7 +20,8 @@ CASES = [
     "DelayManagerTester.test_updates",
     "PipelineTester.simple_run_test",
     "Pipeline.outputs_match_operators_16ant",
-    "Pipeline.outputs_match_operators_64ant_taftp",
+    "Pipeline.outputs_match_operators_64ant_fused",
+    "Pipeline.outputs_match_operators_64of128_taftp",
     "TransposeToDada.beams_match_oracle",
     "FengToDada.reorders_in_place",
 ]

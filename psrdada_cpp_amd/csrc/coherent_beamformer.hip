@@ -949,7 +949,7 @@ struct CbTrArgs {
     // Incoherent-beam partials (cb_tr_k with IBF): [F][ntiles][2 halves][32
     // columns] int32; sample j of a tile has the power over the block's
     // antennas and both pols split over columns 2j, 2j + 1 (its pols) of
-    // both halves (ib_finish_k adds the four).
+    // both halves (ib_finish_run adds the four).
     int* ibp;
 };
 
@@ -2146,10 +2146,17 @@ struct TbtfArgs {
     int n_ttiles, n_ftiles, n_btiles;
 };
 
-__global__ __launch_bounds__(256) void tbtf_k(TbtfArgs a)
+// The first job.nblocks work-groups finish the incoherent beam from the
+// coherent kernel's partials (ib_finish_run) beside the reorder; the rest
+// reorder, numbered from 0.
+__global__ __launch_bounds__(256) void tbtf_k(TbtfArgs a, IbFinishJob job)
 {
     extern __shared__ __align__(16) int8_t tr_lds[];
-    long long bid = blockIdx.x;
+    if ((int)blockIdx.x < job.nblocks) {   // uniform over the work-group
+        ib_finish_run(job, (long long)blockIdx.x * 256 + threadIdx.x);
+        return;
+    }
+    long long bid = (long long)blockIdx.x - job.nblocks;
     const int bt = (int)(bid % a.n_btiles);
     bid /= a.n_btiles;
     const int ftile = (int)(bid % a.n_ftiles);
@@ -2322,13 +2329,14 @@ TbtfArgs tbtf_args(const int8_t* src, int8_t* dst, long long To, int B, int Fo, 
     return a;
 }
 
-hipError_t launch_tbtf(const TbtfArgs& a, hipStream_t s)
+hipError_t launch_tbtf(const TbtfArgs& a, hipStream_t s, const IbFinishJob& job = IbFinishJob{})
 {
     const long long heaps = a.To / a.ohs;
-    const long long nblocks = heaps * a.n_ttiles * a.n_ftiles * a.n_btiles;
+    const long long nblocks = heaps * a.n_ttiles * a.n_ftiles * a.n_btiles + job.nblocks;
     if (nblocks <= 0) return hipSuccess;
+    if (nblocks >= (1ll << 31)) return hipErrorInvalidConfiguration;
     const size_t shm = (size_t)a.tt * a.tp * 4;
-    tbtf_k<<<dim3((unsigned)nblocks), 256, shm, s>>>(a);
+    tbtf_k<<<dim3((unsigned)nblocks), 256, shm, s>>>(a, job);
     return hipGetLastError();
 }
 
@@ -2591,6 +2599,7 @@ struct TaftpView {
     int S, Atot, a0;
     int variant;   // the kernel variant to pick from (cb_taftp_variant)
     int* ibp;      // incoherent-beam partials to leave (cb_ib_fused), or null
+    IbFinishJob ibj;   // their finish, run in the reorder's launch
 };
 
 // Validates and launches one coherent-beam kernel over T samples of voltage
@@ -2729,7 +2738,7 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
         if (e != hipSuccess) return hip_status(e, "cb_tr_k launch");
         e = launch_tbtf(tbtf_args(static_cast<const int8_t*>(kout), static_cast<int8_t*>(out), To, d.nbeams, Fo,
                                   plan->ohs),
-                        s);
+                        s, tv ? tv->ibj : IbFinishJob{});
         if (e != hipSuccess) return hip_status(e, "tbtf_k launch");
         return FBF_OK;
     }
@@ -2884,7 +2893,8 @@ extern "C" int fbf_cb_beamform_taftp(fbf_cb_plan* plan, const void* taftp, size_
     FBF_REQUIRE((reinterpret_cast<uintptr_t>(taftp) & 15) == 0, FBF_ERR_ARG,
                 "fbf_cb_beamform_taftp: buffers must be 16-byte aligned");
     if (T == 0 || cb_taftp_direct(plan, T, nantennas_total, nsamples_per_heap)) {
-        const TaftpView tv{nsamples_per_heap, nantennas_total, start_antenna, cb_taftp_variant(plan, T), nullptr};
+        const TaftpView tv{nsamples_per_heap, nantennas_total, start_antenna, cb_taftp_variant(plan, T), nullptr,
+                           IbFinishJob{}};
         return cb_launch(plan, MODE_INT8, taftp, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf,
                          tbtf_bytes, stream, nullptr, 0, T ? &tv : nullptr);
     }
@@ -2957,11 +2967,12 @@ extern "C" int fbf_cb_ib_beamform_taftp(fbf_cb_plan* plan, fbf_ib_plan* ib_plan,
     hipStream_t s = static_cast<hipStream_t>(stream);
     st = cb_grow(&plan->ibp, &plan->ibp_bytes, (size_t)d.nchans * T * 4 * sizeof(int), s, &part);
     if (st) return st;
-    const TaftpView tv{nsamples_per_heap, d.nantennas, 0, cb_taftp_variant(plan, T), static_cast<int*>(part)};
-    st = cb_launch(plan, MODE_INT8, taftp, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf, tbtf_bytes,
-                   stream, nullptr, 0, &tv);
+    IbFinishJob ibj{};
+    st = ib_finish_job(ib_plan, static_cast<const int*>(part), T, tf, tf_bytes, &ibj);
     if (st) return st;
-    return ib_finish_launch(ib_plan, static_cast<const int*>(part), T, tf, tf_bytes, s);
+    const TaftpView tv{nsamples_per_heap, d.nantennas, 0, cb_taftp_variant(plan, T), static_cast<int*>(part), ibj};
+    return cb_launch(plan, MODE_INT8, taftp, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf, tbtf_bytes,
+                     stream, nullptr, 0, &tv);
 }
 
 extern "C" int fbf_cb_ib_fused(const fbf_cb_plan* plan, const fbf_ib_plan* ib_plan, int nsamples_per_heap,

@@ -121,32 +121,6 @@ __global__ __launch_bounds__(256) void ib_k(IbArgs p, WeightsJob wj)
     }
 }
 
-// The incoherent beam from the coherent beamformer's partials ([F][T / 16]
-// [2 halves][32 columns] int32, sample j of a tile split over columns 2j,
-// 2j + 1 of both halves): one thread per output (window, output channel),
-// the sum over the fscrunch channels, tscrunch samples and the four places
-// in int64, then the same quantiser as ib_k.  Consecutive threads take
-// consecutive output channels, as TF stores them.
-__global__ __launch_bounds__(256) void ib_finish_k(const int* __restrict__ part, int8_t* __restrict__ out,
-                                                   const float* offsets, const float* scales, long long nout,
-                                                   int ntiles, int Fo, int ts, int fs, int quant)
-{
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nout) return;
-    const int fo = (int)(i % Fo);
-    const long long w = i / Fo;
-    long long P = 0;
-    for (int fc = 0; fc < fs; ++fc) {
-        const int* pf = part + (size_t)(fo * fs + fc) * ntiles * 64;
-        for (int k = 0; k < ts; ++k) {
-            const long long t = w * ts + k;
-            const int* q = pf + (size_t)(t >> 4) * 64 + 2 * (t & 15);
-            P += (long long)q[0] + q[1] + q[32] + q[33];
-        }
-    }
-    out[i] = quantise((double)P, offsets[fo], scales[fo], quant);
-}
-
 }  // namespace
 }  // namespace fbf
 
@@ -276,23 +250,23 @@ static int ib_run(fbf_ib_plan* plan, int mode, const void* taftp, size_t taftp_b
     return FBF_OK;
 }
 
-int fbf::ib_finish_launch(const fbf_ib_plan* plan, const int* partials, long long T, void* tf, size_t tf_bytes,
-                          hipStream_t stream)
+int fbf::ib_finish_job(const fbf_ib_plan* plan, const int* partials, long long T, void* tf, size_t tf_bytes,
+                       IbFinishJob* out)
 {
-    FBF_REQUIRE(plan && partials && tf, FBF_ERR_ARG, "fbf_ib: null argument");
+    FBF_REQUIRE(plan && partials && tf && out, FBF_ERR_ARG, "fbf_ib: null argument");
     const fbf_ib_desc& d = plan->d;
     FBF_REQUIRE(T % 16 == 0 && T % d.tscrunch == 0, FBF_ERR_SIZE, "fbf_ib: partials of a partial tile");
+    FBF_REQUIRE(d.tscrunch < 16 ? 16 % d.tscrunch == 0 : d.tscrunch % 16 == 0, FBF_ERR_UNSUPPORTED,
+                "fbf_ib: tscrunch must divide 16 or be a multiple of it");
     const int Fo = d.nchans / d.fscrunch;
-    const long long nout = T / d.tscrunch * Fo;
+    const long long nwin = T / d.tscrunch;
+    const long long nout = nwin * Fo;
     FBF_REQUIRE(tf_bytes == (size_t)nout, FBF_ERR_SIZE,
                 "fbf_ib: output size " + std::to_string(tf_bytes) + " != expected " + std::to_string(nout));
-    if (nout == 0) return FBF_OK;
-    const long long nblocks = (nout + 255) / 256;
-    FBF_REQUIRE(nblocks < (1ll << 31) && T / 16 < (1ll << 31), FBF_ERR_UNSUPPORTED, "fbf_ib: grid too large");
-    ib_finish_k<<<dim3((unsigned)nblocks), 256, 0, stream>>>(partials, static_cast<int8_t*>(tf), plan->d_levels,
-                                                               plan->d_levels + Fo, nout, (int)(T / 16), Fo,
-                                                               d.tscrunch, d.fscrunch, d.quant_mode);
-    FBF_LAUNCH_CHECK("ib_finish_k");
+    const long long nblocks = (4 * nout + 255) / 256;
+    FBF_REQUIRE(nblocks < (1ll << 30) && T / 16 < (1ll << 31), FBF_ERR_UNSUPPORTED, "fbf_ib: grid too large");
+    *out = IbFinishJob{partials, static_cast<int8_t*>(tf), plan->d_levels, plan->d_levels + Fo, nwin, (int)(T / 16),
+                       Fo, d.tscrunch, d.fscrunch, d.quant_mode, (int)nblocks};
     return FBF_OK;
 }
 

@@ -6,15 +6,18 @@ A step is one DADA block of the single-subarray workload (64 antennas,
 beams, tscrunch 16, plus the incoherent beam), with the voltages already
 resident in HBM (three distinct blocks, 403 MB together, rotate so that the
 input is not served from the 256 MB Infinity Cache):
-    delay -> weights at the block's epoch (FP64 phase), every block as
-        Pipeline.cu:164 does, generated in the incoherent beam's launch
-    incoherent beam from TAFTP -> TF int8
     coherent beamformer straight from TAFTP (staged and transposed in LDS,
         no split-transpose pass; int8 MFMA, detect, scrunch, quantise)
-        -> TBTF int8
-all on one stream (--path ftpa: the reference's order instead, a
-split-transpose pass to FTPA with the incoherent beam in it, then the
-coherent beamformer on FTPA).  Prints one JSON line.
+        -> TBTF int8, the same kernel leaving the incoherent beam's
+        per-sample partials
+    TBTF reorder, whose launch also finishes the incoherent beam (TF int8)
+        and generates the next block's weights (delay -> weights at that
+        block's epoch, FP64 phase: every block's weights are generated
+        once, as Pipeline.cu:164 does at the block's start)
+all on one stream (--path taftp: weights + incoherent beam in one launch,
+then the coherent beamformer from TAFTP; --path ftpa: the reference's
+order, a split-transpose pass to FTPA with the incoherent beam and the
+weights in it, then the coherent beamformer on FTPA).  Prints one JSON line.
 
 value        = x real-time: block duration / wall time per block (K blocks
                timed back to back, device synchronised on both sides).
@@ -140,6 +143,12 @@ def main():
         raise RuntimeError("the coherent-beam plan does not read this block straight from TAFTP")
     if fused_path and not cb.ib_fused(ib, T):
         raise RuntimeError("the plans do not form the incoherent beam inside the coherent kernel")
+    # Epoch of the weights last written to d_w (for the CPU baseline's check).
+    w_epoch = [0.0]
+    if fused_path:
+        # Block k's weights come from block k - 1's reorder launch; the first
+        # block's are generated here.
+        wp.generate(d_delays, 0.0, d_w, stream=s)
 
     def step(k, ev=None, weights_every=1):
         """One block on stream s.  A block that takes new weights generates
@@ -154,6 +163,7 @@ def main():
             ev[0].record(s)
             if new_w:
                 wp.generate(d_delays, epoch, d_w, stream=s)
+                w_epoch[0] = epoch
             ev[1].record(s)
             if fused_path:
                 pass
@@ -165,24 +175,33 @@ def main():
                 capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
             ev[2].record(s)
         elif fused_path:
-            if new_w:
-                wp.generate(d_delays, epoch, d_w, stream=s)
+            pass
         elif taftp_path:
             if new_w:
                 ib.beamform_weights(d_taftp, d_ib, wp, d_delays, epoch, d_w, stream=s)
+                w_epoch[0] = epoch
             else:
                 ib.beamform(d_taftp, d_ib, stream=s)
         elif fused and new_w:
             capi.split_transpose_ib_weights(d_taftp, d_ftpa, F, ib, d_ib, wp, d_delays, epoch, d_w, stream=s)
+            w_epoch[0] = epoch
         else:
             if new_w:
                 wp.generate(d_delays, epoch, d_w, stream=s)
+                w_epoch[0] = epoch
             if fused:
                 capi.split_transpose_ib(d_taftp, d_ftpa, F, ib, d_ib, stream=s)
             else:
                 capi.split_transpose(d_taftp, d_ftpa, cfg.total_nantennas, A, 0, F, stream=s)
         if fused_path:
-            cb.ib_beamform_taftp(ib, d_taftp, d_w, d_cb, d_ib, stream=s)
+            # The reorder's launch generates the next block's weights when
+            # that block takes new ones.
+            if (k + 1) % weights_every == 0:
+                cb.ib_beamform_taftp_next(ib, d_taftp, d_w, d_cb, d_ib, wp, d_delays, epoch + cfg.block_seconds,
+                                          d_w, stream=s)
+                w_epoch[0] = epoch + cfg.block_seconds
+            else:
+                cb.ib_beamform_taftp(ib, d_taftp, d_w, d_cb, d_ib, stream=s)
         elif taftp_path:
             cb.beamform_taftp(d_taftp, d_w, d_cb, nantennas_total=cfg.total_nantennas, stream=s)
         else:
@@ -252,6 +271,7 @@ def main():
         fe = [capi.Event() for _ in range(R + 1)]
         fe[0].record(s)
         for r in range(R):
+            w_epoch[0] = cfg.block_seconds * r
             if taftp_path:
                 ib.beamform_weights(d_taftps[r % NBLK], d_ib, wp, d_delays, cfg.block_seconds * r, d_w, stream=s)
             else:
@@ -311,8 +331,10 @@ def main():
             "cb_tscrunch": sub.tscrunch, "cb_fscrunch": sub.fscrunch,
             "ib_tscrunch": cfg.ib_tscrunch, "ib_fscrunch": cfg.ib_fscrunch,
             "block_ms": round(block_s * 1e3, 3),
-            "stages": ("weights (every block), then the coherent and incoherent beams in one pass over TAFTP "
-                       "(the coherent kernel leaves per-sample IB partials; no split-transpose pass)" if fused_path
+            "stages": ("the coherent and incoherent beams in one pass over TAFTP (the coherent kernel leaves "
+                       "per-sample IB partials; no split-transpose pass), then the TBTF reorder, whose launch also "
+                       "finishes the IB and generates the next block's weights (every block's weights once)"
+                       if fused_path
                        else "weights (every block) and the incoherent beam in one launch, then the coherent beams "
                        "straight from TAFTP (no split-transpose pass)" if taftp_path
                        else "weights (every block), split-transpose and IB in one launch, then CB" if fused
@@ -335,19 +357,23 @@ def main():
         "kernel_ms": kernel_ms,
         "kernel_hbm_gbs": kernel_gbs,
         "roofline": {
-            "kernel": "coherent beamformer (cb_tr_k int8 MFMA + detect/quantise, then tbtf_k reorder)",
+            "kernel": ("coherent beamformer (cb_tr_k int8 MFMA + detect/quantise from TAFTP, with the incoherent "
+                       "beam's partials; then tbtf_k reorder + IB finish)" if fused_path else
+                       "coherent beamformer (cb_tr_k int8 MFMA + detect/quantise, then tbtf_k reorder)"),
             "bound": "mfma",
             "achieved": round(cb_tops, 2),
             "peak": wl.PEAK_INT8_TOPS,
             "unit": "TOP/s",
             "frac": round(cb_tops / wl.PEAK_INT8_TOPS, 4),
-            "traffic": profiled_traffic(),
-            "traffic_source": "profile constant: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of the CB kernels "
-                              "on this workload, scaled by a plain copy's readings, from "
-                              "profiles/cb_64ant_64ch_512beams_hbm_traffic.txt (not measured in this run)",
+            "traffic": profiled_traffic(args.path),
+            "traffic_source": ("profile constant: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of the window's "
+                               "kernels on this workload, scaled by a plain copy's readings, from "
+                               + os.path.relpath(TRAFFIC_PROFILES[args.path], ROOT) + " (not measured in this run)"
+                               if args.path in TRAFFIC_PROFILES else "no committed profile for this path"),
             "ops_per_launch": ops,
-            "window": ("HIP events around fbf_cb_ib_beamform_taftp on its stream: cb_tr_k (staging the TAFTP "
-                       "block itself, with the IB partials) plus the tbtf_k reorder and ib_finish_k" if fused_path
+            "window": ("HIP events around fbf_cb_ib_beamform_taftp_next on its stream: cb_tr_k (staging the "
+                       "TAFTP block itself, with the IB partials), then tbtf_k's launch (the reorder, the IB finish "
+                       "and the next block's weights)" if fused_path
                        else "HIP events around fbf_cb_beamform_taftp on its stream: cb_tr_k (staging the TAFTP "
                        "block itself) plus the tbtf_k reorder" if taftp_path else
                        "HIP events around fbf_cb_beamform on its stream: cb_tr_k plus the tbtf_k reorder"),
@@ -357,8 +383,7 @@ def main():
     gpu_weights = d_w.download()      # the last block's weights, for the CPU baseline's check
     # the epoch of the last weights written to d_w (the fused launches above
     # when the block uses them, else the per-kernel pass)
-    gpu_weights_epoch = cfg.block_seconds * ((R - 1) if (fused or (taftp_path and not fused_path))
-                                             else (args.warmup + 2 * K + R - 1))
+    gpu_weights_epoch = w_epoch[0]
     wp.close()
     cb.close()
     ib.close()
@@ -506,16 +531,23 @@ def config1_leg(warmup, reps=200):
             "x_real_time": round(cfg.block_seconds * 1e3 / ms, 1), "block_ms": round(cfg.block_seconds * 1e3, 3)}
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cb_64ant_64ch_512beams_hbm_traffic.txt")
+TRAFFIC_PROFILES = {
+    "ftpa": os.path.join(ROOT, "profiles", "cb_64ant_64ch_512beams_hbm_traffic.txt"),
+    "fused": os.path.join(ROOT, "profiles", "cb_ib_fused_64ant_64ch_512beams_hbm_traffic.txt"),
+}
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the coherent-beam kernels (beamformer +
-    reorder) on this workload, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+def profiled_traffic(path):
+    """HBM bytes per launch of the window's kernels (coherent beamformer +
+    reorder, with the incoherent beam's partials and finish on the fused
+    path) on this workload, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary committed under profiles/ (counter readings scaled by what they
-    read for a plain 1 GiB copy); None when the summary is absent."""
+    read for a plain 1 GiB copy); None when there is no summary for the
+    path."""
+    if path not in TRAFFIC_PROFILES:
+        return None
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(TRAFFIC_PROFILES[path]) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return int(line.split(":", 1)[1])

@@ -171,8 +171,11 @@ int fbf_cb_beamform_ws(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
 int fbf_cb_beamform_noalloc(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes,
                             const void* fba_weights, size_t weights_bytes,
                             void* tbtf, size_t tbtf_bytes, fbf_stream_t stream);
-/* Sizes the plan's staging buffer for blocks of nsamples samples (call before
- * capturing a stream that runs the plan). */
+/* Sizes the plan's staging buffer and the incoherent-beam partials of
+ * fbf_cb_ib_beamform_taftp for blocks of nsamples samples (call before
+ * capturing a stream that runs the plan).  The FTPA copy that
+ * fbf_cb_beamform_taftp makes when it cannot stage the TAFTP block itself is
+ * sized by its first call outside capture. */
 int fbf_cb_plan_reserve(fbf_cb_plan* plan, long long nsamples);
 /* Beamform the plan's antennas as a subarray of a full-antenna FTPA block:
  * ftpa_block holds block_nantennas antennas per column and this subarray's
@@ -343,6 +346,22 @@ int fbf_ib_beamform_weights(fbf_ib_plan* plan, const void* taftp, size_t taftp_b
                             size_t tf_bytes, fbf_weights_plan* weights_plan, const void* delays,
                             size_t delays_bytes, double epoch, void* weights, size_t weights_bytes,
                             fbf_stream_t stream);
+/* fbf_cb_ib_beamform_taftp, and then fbf_weights_generate(weights_plan,
+ * delays, delays_bytes, next_epoch, next_weights, next_weights_bytes, NULL,
+ * stream): the next block's weights.  On the one-pass path the weights are
+ * generated by the first work-groups of the TBTF reorder's launch, which
+ * runs after the coherent kernel has read this block's weights, so
+ * next_weights may be fba_weights itself.  Outputs equal those of the calls
+ * in that order.  No counterpart in the reference, whose Pipeline generates
+ * each block's weights at the start of that block (Pipeline.cu:164); a
+ * caller reusing the next weights must regenerate them if the delay model
+ * changes in between (the C++ Pipeline does). */
+int fbf_cb_ib_beamform_taftp_next(fbf_cb_plan* plan, fbf_ib_plan* ib_plan, const void* taftp,
+                                  size_t taftp_bytes, int nsamples_per_heap, const void* fba_weights,
+                                  size_t weights_bytes, void* tbtf, size_t tbtf_bytes, void* tf, size_t tf_bytes,
+                                  fbf_weights_plan* weights_plan, const void* delays, size_t delays_bytes,
+                                  double next_epoch, void* next_weights, size_t next_weights_bytes,
+                                  fbf_stream_t stream);
 
 /* ---- batched subarrays on one block (no counterpart in the reference:
  * it compiles one subarray per process, fbfuse_constants.hpp:12-14) ---- */

@@ -115,6 +115,8 @@ _SIGS = {
     "fbf_cb_taftp_direct": [vp, i32, i32, C.c_longlong, C.POINTER(C.c_int)],
     "fbf_cb_ib_beamform_taftp": [vp, vp, vp, sz, i32, vp, sz, vp, sz, vp, sz, vp],
     "fbf_cb_ib_fused": [vp, vp, i32, C.c_longlong, C.POINTER(C.c_int)],
+    "fbf_cb_ib_beamform_taftp_next": [vp, vp, vp, sz, i32, vp, sz, vp, sz, vp, sz, vp, vp, sz, C.c_double, vp, sz,
+                                      vp],
     "fbf_cb_plan_set_chunk": [vp, i32],
     "fbf_cb_autotune": [vp, C.c_longlong, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)],
     "fbf_batch_create": [i32, i32, i32, i32, i32, C.POINTER(SubarrayDesc), C.POINTER(C.c_void_p),
@@ -453,6 +455,15 @@ class CbPlan:
         call("fbf_cb_ib_beamform_taftp", self.handle, ib_plan.handle, taftp.ptr, taftp.nbytes,
              int(nsamples_per_heap), weights.ptr, weights.nbytes, out.ptr, out.nbytes, tf.ptr, tf.nbytes,
              _s(stream))
+
+    def ib_beamform_taftp_next(self, ib_plan, taftp, weights, out, tf, weights_plan, delays, next_epoch,
+                               next_weights, nsamples_per_heap=256, stream=None):
+        """ib_beamform_taftp, then the next block's weights (at next_epoch,
+        into next_weights, which may be `weights`)."""
+        call("fbf_cb_ib_beamform_taftp_next", self.handle, ib_plan.handle, taftp.ptr, taftp.nbytes,
+             int(nsamples_per_heap), weights.ptr, weights.nbytes, out.ptr, out.nbytes, tf.ptr, tf.nbytes,
+             weights_plan.handle, delays.ptr, delays.nbytes, float(next_epoch), next_weights.ptr,
+             next_weights.nbytes, _s(stream))
 
     def ib_fused(self, ib_plan, nsamples, nsamples_per_heap=256):
         """Whether ib_beamform_taftp forms the incoherent beam inside the

@@ -140,18 +140,30 @@ void Pipeline::process(VoltageVectorType const& taftp, PowerVectorType& tbtf, Po
         FBF_ERROR_CHECK(fbf_cb_ib_fused(_cb_plan, _ib_plan, (int)_config.nsamples_per_heap(),
                                         (long long)_nsamples_per_dada_block, &one_pass));
     if (one_pass) {
-        // The weights, then both beams in one pass over the TAFTP block (the
-        // coherent kernel leaves the incoherent beam's partials): the same
-        // outputs as the reference's order (Pipeline.cu:164-170).
-        FBF_ERROR_CHECK(fbf_weights_generate(_weights_plan, delays.data(), delays.bytes(),
-                                             (double)_unix_timestamp, _weights.data(), _weights.bytes(), nullptr,
-                                             _processing_stream));
-        FBF_ERROR_CHECK(fbf_cb_ib_beamform_taftp(_cb_plan, _ib_plan, taftp.data(), taftp.bytes(),
-                                                 (int)_config.nsamples_per_heap(), _weights.data(), _weights.bytes(),
-                                                 tbtf.data(), tbtf.bytes(), tf.data(), tf.bytes(),
-                                                 _processing_stream));
+        // Both beams in one pass over the TAFTP block (the coherent kernel
+        // leaves the incoherent beam's partials), and the reorder's launch
+        // generates the next block's weights.  This block's weights came from
+        // the previous block's launch when they were made for this epoch with
+        // the delay model still in force; else they are generated first.  The
+        // outputs are those of the reference's order (Pipeline.cu:164-170).
+        const double epoch = (double)_unix_timestamp;
+        if (!(_next_weights_valid && _next_weights_epoch == epoch &&
+              _next_weights_updates == _delay_manager->update_count()))
+            FBF_ERROR_CHECK(fbf_weights_generate(_weights_plan, delays.data(), delays.bytes(), epoch,
+                                                 _weights.data(), _weights.bytes(), nullptr, _processing_stream));
+        const double next_epoch =
+            (double)(_sync_time + (_sample_clock_start + ((_call_count - 1) * _sample_clock_tick_per_block)) /
+                                      _sample_clock);
+        FBF_ERROR_CHECK(fbf_cb_ib_beamform_taftp_next(
+            _cb_plan, _ib_plan, taftp.data(), taftp.bytes(), (int)_config.nsamples_per_heap(), _weights.data(),
+            _weights.bytes(), tbtf.data(), tbtf.bytes(), tf.data(), tf.bytes(), _weights_plan, delays.data(),
+            delays.bytes(), next_epoch, _weights.data(), _weights.bytes(), _processing_stream));
+        _next_weights_valid = true;
+        _next_weights_epoch = next_epoch;
+        _next_weights_updates = _delay_manager->update_count();
         return;
     }
+    _next_weights_valid = false;
     FBF_ERROR_CHECK(fbf_cb_taftp_direct(_cb_plan, (int)_config.total_nantennas(), (int)_config.nsamples_per_heap(),
                                         (long long)_nsamples_per_dada_block, &direct));
     if (direct) {

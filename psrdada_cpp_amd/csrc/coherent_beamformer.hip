@@ -2146,17 +2146,20 @@ struct TbtfArgs {
     int n_ttiles, n_ftiles, n_btiles;
 };
 
-// The first job.nblocks work-groups finish the incoherent beam from the
-// coherent kernel's partials (ib_finish_run) beside the reorder; the rest
-// reorder, numbered from 0.
-__global__ __launch_bounds__(256) void tbtf_k(TbtfArgs a, IbFinishJob job)
+// The first wj.nblocks work-groups generate weights (the next block's,
+// weights_job_run), the next job.nblocks finish the incoherent beam from the
+// coherent kernel's partials (ib_finish_run), both beside the reorder; the
+// rest reorder, numbered from 0.
+__global__ __launch_bounds__(256) void tbtf_k(TbtfArgs a, IbFinishJob job, WeightsJob wj)
 {
     extern __shared__ __align__(16) int8_t tr_lds[];
-    if ((int)blockIdx.x < job.nblocks) {   // uniform over the work-group
-        ib_finish_run(job, (long long)blockIdx.x * 256 + threadIdx.x);
+    if (weights_job_run(wj, threadIdx.x)) return;   // uniform over the work-group
+    const int b1 = (int)blockIdx.x - wj.nblocks;
+    if (b1 < job.nblocks) {
+        ib_finish_run(job, (long long)b1 * 256 + threadIdx.x);
         return;
     }
-    long long bid = (long long)blockIdx.x - job.nblocks;
+    long long bid = (long long)b1 - job.nblocks;
     const int bt = (int)(bid % a.n_btiles);
     bid /= a.n_btiles;
     const int ftile = (int)(bid % a.n_ftiles);
@@ -2329,14 +2332,15 @@ TbtfArgs tbtf_args(const int8_t* src, int8_t* dst, long long To, int B, int Fo, 
     return a;
 }
 
-hipError_t launch_tbtf(const TbtfArgs& a, hipStream_t s, const IbFinishJob& job = IbFinishJob{})
+hipError_t launch_tbtf(const TbtfArgs& a, hipStream_t s, const IbFinishJob& job = IbFinishJob{},
+                       const WeightsJob& wj = WeightsJob{})
 {
     const long long heaps = a.To / a.ohs;
-    const long long nblocks = heaps * a.n_ttiles * a.n_ftiles * a.n_btiles + job.nblocks;
+    const long long nblocks = heaps * a.n_ttiles * a.n_ftiles * a.n_btiles + job.nblocks + wj.nblocks;
     if (nblocks <= 0) return hipSuccess;
     if (nblocks >= (1ll << 31)) return hipErrorInvalidConfiguration;
     const size_t shm = (size_t)a.tt * a.tp * 4;
-    tbtf_k<<<dim3((unsigned)nblocks), 256, shm, s>>>(a, job);
+    tbtf_k<<<dim3((unsigned)nblocks), 256, shm, s>>>(a, job, wj);
     return hipGetLastError();
 }
 
@@ -2600,6 +2604,7 @@ struct TaftpView {
     int variant;   // the kernel variant to pick from (cb_taftp_variant)
     int* ibp;      // incoherent-beam partials to leave (cb_ib_fused), or null
     IbFinishJob ibj;   // their finish, run in the reorder's launch
+    WeightsJob wj;     // weights generated in the reorder's launch (the next block's)
 };
 
 // Validates and launches one coherent-beam kernel over T samples of voltage
@@ -2738,7 +2743,7 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
         if (e != hipSuccess) return hip_status(e, "cb_tr_k launch");
         e = launch_tbtf(tbtf_args(static_cast<const int8_t*>(kout), static_cast<int8_t*>(out), To, d.nbeams, Fo,
                                   plan->ohs),
-                        s, tv ? tv->ibj : IbFinishJob{});
+                        s, tv ? tv->ibj : IbFinishJob{}, tv ? tv->wj : WeightsJob{});
         if (e != hipSuccess) return hip_status(e, "tbtf_k launch");
         return FBF_OK;
     }
@@ -2894,7 +2899,7 @@ extern "C" int fbf_cb_beamform_taftp(fbf_cb_plan* plan, const void* taftp, size_
                 "fbf_cb_beamform_taftp: buffers must be 16-byte aligned");
     if (T == 0 || cb_taftp_direct(plan, T, nantennas_total, nsamples_per_heap)) {
         const TaftpView tv{nsamples_per_heap, nantennas_total, start_antenna, cb_taftp_variant(plan, T), nullptr,
-                           IbFinishJob{}};
+                           IbFinishJob{}, WeightsJob{}};
         return cb_launch(plan, MODE_INT8, taftp, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf,
                          tbtf_bytes, stream, nullptr, 0, T ? &tv : nullptr);
     }
@@ -2932,10 +2937,22 @@ static bool cb_ib_fused(const fbf_cb_plan* plan, const IbFuseParams& ibp, long l
            ibp.nchans == d.nchans && ibp.nsamples_per_heap == S;
 }
 
-extern "C" int fbf_cb_ib_beamform_taftp(fbf_cb_plan* plan, fbf_ib_plan* ib_plan, const void* taftp,
-                                        size_t taftp_bytes, int nsamples_per_heap, const void* fba_weights,
-                                        size_t weights_bytes, void* tbtf, size_t tbtf_bytes, void* tf,
-                                        size_t tf_bytes, fbf_stream_t stream)
+// Both beams of a TAFTP block in one pass (or one after the other where the
+// plans do not allow it); `wjob`, when it has work-groups, generates weights
+// in the reorder's launch, else `next` (a fbf_weights_generate call) runs
+// after the beams.
+struct NextWeights {
+    fbf_weights_plan* plan;
+    const void* delays;
+    size_t delays_bytes;
+    double epoch;
+    void* weights;
+    size_t weights_bytes;
+};
+
+static int cb_ib_run(fbf_cb_plan* plan, fbf_ib_plan* ib_plan, const void* taftp, size_t taftp_bytes,
+                     int nsamples_per_heap, const void* fba_weights, size_t weights_bytes, void* tbtf,
+                     size_t tbtf_bytes, void* tf, size_t tf_bytes, fbf_stream_t stream, const NextWeights* next)
 {
     FBF_REQUIRE(plan && ib_plan, FBF_ERR_ARG, "fbf_cb_ib_beamform_taftp: null plan");
     FBF_REQUIRE(taftp && fba_weights && tbtf && tf, FBF_ERR_ARG, "fbf_cb_ib_beamform_taftp: null buffer");
@@ -2948,12 +2965,27 @@ extern "C" int fbf_cb_ib_beamform_taftp(fbf_cb_plan* plan, fbf_ib_plan* ib_plan,
     FBF_REQUIRE(taftp_bytes % hg_bytes == 0, FBF_ERR_SIZE,
                 "fbf_cb_ib_beamform_taftp: input is not a whole number of heap groups of the plan's antennas");
     const long long T = (long long)(taftp_bytes / hg_bytes) * nsamples_per_heap;
+    WeightsJob wj{};
+    if (next) {
+        // Checked before anything is launched.
+        st = weights_launch_params(next->plan, next->delays, next->delays_bytes, next->epoch, next->weights,
+                                   next->weights_bytes, &wj.w);
+        if (st) return st;
+        FBF_REQUIRE(next->weights_bytes == weights_bytes, FBF_ERR_SIZE,
+                    "fbf_cb_ib_beamform_taftp_next: the next weights differ in size from the block's");
+        FBF_REQUIRE((unsigned long long)wj.w.gx * wj.w.gy < (1ull << 30), FBF_ERR_UNSUPPORTED,
+                    "fbf_cb_ib_beamform_taftp_next: too many weights work-groups");
+        wj.nblocks = (int)(wj.w.gx * wj.w.gy);
+    }
     if (T == 0 || !cb_ib_fused(plan, ibp, T, nsamples_per_heap)) {
-        // The two beamformers one after the other.
+        // The two beamformers one after the other, then the next weights.
         st = fbf_ib_beamform(ib_plan, taftp, taftp_bytes, tf, tf_bytes, stream);
         if (st) return st;
-        return fbf_cb_beamform_taftp(plan, taftp, taftp_bytes, d.nantennas, 0, nsamples_per_heap, fba_weights,
-                                     weights_bytes, tbtf, tbtf_bytes, stream);
+        st = fbf_cb_beamform_taftp(plan, taftp, taftp_bytes, d.nantennas, 0, nsamples_per_heap, fba_weights,
+                                   weights_bytes, tbtf, tbtf_bytes, stream);
+        if (st || !next) return st;
+        return fbf_weights_generate(next->plan, next->delays, next->delays_bytes, next->epoch, next->weights,
+                                    next->weights_bytes, nullptr, stream);
     }
     size_t tf_expect = 0;
     st = fbf_ib_output_bytes(ib_plan, taftp_bytes, &tf_expect);
@@ -2970,9 +3002,31 @@ extern "C" int fbf_cb_ib_beamform_taftp(fbf_cb_plan* plan, fbf_ib_plan* ib_plan,
     IbFinishJob ibj{};
     st = ib_finish_job(ib_plan, static_cast<const int*>(part), T, tf, tf_bytes, &ibj);
     if (st) return st;
-    const TaftpView tv{nsamples_per_heap, d.nantennas, 0, cb_taftp_variant(plan, T), static_cast<int*>(part), ibj};
+    const TaftpView tv{nsamples_per_heap, d.nantennas, 0, cb_taftp_variant(plan, T), static_cast<int*>(part), ibj,
+                       wj};
     return cb_launch(plan, MODE_INT8, taftp, T, 2 * d.nantennas, 0, fba_weights, weights_bytes, tbtf, tbtf_bytes,
                      stream, nullptr, 0, &tv);
+}
+
+extern "C" int fbf_cb_ib_beamform_taftp(fbf_cb_plan* plan, fbf_ib_plan* ib_plan, const void* taftp,
+                                        size_t taftp_bytes, int nsamples_per_heap, const void* fba_weights,
+                                        size_t weights_bytes, void* tbtf, size_t tbtf_bytes, void* tf,
+                                        size_t tf_bytes, fbf_stream_t stream)
+{
+    return cb_ib_run(plan, ib_plan, taftp, taftp_bytes, nsamples_per_heap, fba_weights, weights_bytes, tbtf,
+                     tbtf_bytes, tf, tf_bytes, stream, nullptr);
+}
+
+extern "C" int fbf_cb_ib_beamform_taftp_next(fbf_cb_plan* plan, fbf_ib_plan* ib_plan, const void* taftp,
+                                             size_t taftp_bytes, int nsamples_per_heap, const void* fba_weights,
+                                             size_t weights_bytes, void* tbtf, size_t tbtf_bytes, void* tf,
+                                             size_t tf_bytes, fbf_weights_plan* weights_plan, const void* delays,
+                                             size_t delays_bytes, double next_epoch, void* next_weights,
+                                             size_t next_weights_bytes, fbf_stream_t stream)
+{
+    const NextWeights next{weights_plan, delays, delays_bytes, next_epoch, next_weights, next_weights_bytes};
+    return cb_ib_run(plan, ib_plan, taftp, taftp_bytes, nsamples_per_heap, fba_weights, weights_bytes, tbtf,
+                     tbtf_bytes, tf, tf_bytes, stream, &next);
 }
 
 extern "C" int fbf_cb_ib_fused(const fbf_cb_plan* plan, const fbf_ib_plan* ib_plan, int nsamples_per_heap,
@@ -3029,7 +3083,11 @@ extern "C" int fbf_cb_plan_reserve(fbf_cb_plan* plan, long long nsamples)
     const size_t bytes = (size_t)(nsamples / plan->d.tscrunch) * plan->d.nbeams * (plan->d.nchans / plan->d.fscrunch);
     if (bytes == 0) return FBF_OK;
     void* p = nullptr;
-    return cb_staging(plan, bytes, nullptr, &p);
+    const int st = cb_staging(plan, bytes, nullptr, &p);
+    if (st) return st;
+    // The incoherent-beam partials of fbf_cb_ib_beamform_taftp (16 bytes per
+    // channel and sample).
+    return cb_grow(&plan->ibp, &plan->ibp_bytes, (size_t)plan->d.nchans * nsamples * 4 * sizeof(int), nullptr, &p);
 }
 
 extern "C" int fbf_cb_beam_sums(fbf_cb_plan* plan, const void* ftpa, size_t ftpa_bytes, const void* fba_weights,

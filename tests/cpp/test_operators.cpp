@@ -570,8 +570,12 @@ void pipeline_simple_run()
 // through SplitTranspose + CoherentBeamformer, the incoherent beam through
 // IncoherentBeamformer.  At 16 antennas the pipeline split-transposes (the
 // reference's order); at 64 it forms both beams in one pass over the TAFTP
-// block; for 64 antennas of a 128-antenna block it forms the coherent beams
+// block, each block's reorder launch generating the next block's weights;
+// for 64 antennas of a 128-antenna block it forms the coherent beams
 // straight from the TAFTP block, the weights in the incoherent beam's launch.
+// The delay model changes before the call that processes block 2, whose
+// weights the one-pass route had already generated from the old model: the
+// pipeline must generate them again.
 struct CaptureSink {
     std::vector<std::vector<int8_t>> blocks;
     std::string header;
@@ -608,6 +612,9 @@ void pipeline_outputs_match_operators(int Atot, int A, int a0, int B)
     FBF_ERROR_CHECK(fbf_synth_uniform_f32(77, -1e-6f, 1e-6f, offs.data(), nd));
     for (std::size_t i = 0; i < nd; ++i) simulator.delays()[i] = float2{1e-9f, offs[i]};
     simulator.update_delays();
+    std::vector<float2> delays_before(simulator.delays(), simulator.delays() + nd);
+    std::vector<float> offs2(nd);
+    FBF_ERROR_CHECK(fbf_synth_uniform_f32(78, -1e-6f, 1e-6f, offs2.data(), nd));
 
     const int nhg = 8;
     const std::size_t block_bytes = (std::size_t)nhg * Atot * 16 * 256 * 2 * 2;
@@ -637,6 +644,10 @@ void pipeline_outputs_match_operators(int Atot, int A, int a0, int B)
         RawBytes in_rb(static_cast<char*>(in), block_bytes, block_bytes);
         for (int i = 0; i < nblocks; ++i) {
             std::memcpy(in, inputs[i].data(), block_bytes);
+            if (i == 3) {   // call 3 processes block 2
+                for (std::size_t j = 0; j < nd; ++j) simulator.delays()[j] = float2{-2e-9f, offs2[j]};
+                simulator.update_delays();
+            }
             pipeline(in_rb);
             if (i >= 1) ts[i - 1] = pipeline.unix_timestamp();
         }
@@ -673,11 +684,11 @@ void pipeline_outputs_match_operators(int Atot, int A, int a0, int B)
     WeightsManager wm(config, stream);
     EXPECT(ib_sink.blocks.size() == (std::size_t)(nblocks - 2));
     std::vector<float2> dh(simulator.delays(), simulator.delays() + nd);
-    WeightsManager::DelayVectorType dd(dh);
+    WeightsManager::DelayVectorType dd_before(delays_before), dd_after(dh);
     for (int i = 0; i < nblocks - 2; ++i) {
         SplitTranspose::VoltageType taftp(inputs[i]), ftpa;
         st.transpose(taftp, ftpa, stream);
-        auto const& w = wm.weights(dd, (double)ts[i]);
+        auto const& w = wm.weights(i < 2 ? dd_before : dd_after, (double)ts[i]);
         CoherentBeamformer::PowerVectorType out;
         cb.beamform(ftpa, w, out, stream);
         std::vector<int8_t> expect = out.to_host();

@@ -251,3 +251,49 @@ def test_cb_ib_fused_minus128_weights_exact(gpu, oracle):
     w[F - 1, B - 7, 0, 1] = -128
     w[2, 100, :, :] = -128
     _check_cb_ib(gpu, oracle, A, F, nhg, B, 16, 16, 1, 256, 0, True, 2700, weights=w.reshape(-1))
+
+
+@pytest.mark.parametrize("A,B,variant,alias", [(64, 512, 0, True), (64, 256, 0, False), (64, 256, 9, True)])
+def test_cb_ib_next_weights(gpu, oracle, A, B, variant, alias):
+    """fbf_cb_ib_beamform_taftp_next equals fbf_cb_ib_beamform_taftp followed
+    by fbf_weights_generate at the next epoch: beams from this block's
+    weights, then the next block's weights, also when they overwrite this
+    block's (alias) and on the unfused path (variant 9)."""
+    capi = gpu
+    F, nhg, ts, S = 8, 2, 16, 256
+    T = nhg * S
+    taftp = gaussian_int8(2800 + B, nhg * A * F * S * 4, 16.0)
+    off, sc = _levels(oracle, A, ts, 1, F)
+    _, _, ibo, ibs = oracle.power_levels(16.0, 24.0, A, ts, 1, A, 16, 1)
+    plan = capi.CbPlan(A, B, F, ts, 1, off, sc, nsamples_per_heap_out=T // ts)
+    if variant:
+        plan.set_variant(variant)
+    ib = capi.IbPlan(A, F, 16, 1, np.full(F, ibo, np.float32), np.full(F, ibs, np.float32))
+    assert plan.ib_fused(ib, T) == (variant == 0)
+    freqs = capi.channel_frequencies_f64(1284e6, 856e6, F)
+    wp = capi.WeightsPlan(A, B, F, freqs)
+    delays = np.zeros((B * A, 2), np.float32)
+    delays[:, 0] = capi.synth_uniform_f32(2801, -1e-9, 1e-9, B * A)
+    delays[:, 1] = capi.synth_uniform_f32(2802, -1e-6, 1e-6, B * A)
+    d_t, d_d = _dev(capi, taftp), _dev(capi, delays.reshape(-1))
+    d_w = capi.DeviceBuffer(F * B * A * 2)
+    wp.generate(d_d, 3.0, d_w)
+    w_now = d_w.download()
+    d_wn = capi.DeviceBuffer(F * B * A * 2)
+    wp.generate(d_d, 3.0 + 0.0392, d_wn)
+    w_next = d_wn.download()
+    d_cb, d_tf = capi.DeviceBuffer(F * (T // ts) * B), capi.DeviceBuffer(ib.output_bytes(d_t.nbytes))
+    d_cb2, d_tf2 = capi.DeviceBuffer(F * (T // ts) * B), capi.DeviceBuffer(ib.output_bytes(d_t.nbytes))
+    plan.ib_beamform_taftp(ib, d_t, d_w, d_cb, d_tf)
+    d_out_w = d_w if alias else capi.DeviceBuffer(F * B * A * 2)
+    d_cb2.fill(0x5A)
+    d_tf2.fill(0x5A)
+    plan.ib_beamform_taftp_next(ib, d_t, d_w, d_cb2, d_tf2, wp, d_d, 3.0 + 0.0392, d_out_w)
+    np.testing.assert_array_equal(d_cb2.download(), d_cb.download())
+    np.testing.assert_array_equal(d_tf2.download(), d_tf.download())
+    np.testing.assert_array_equal(d_out_w.download(), w_next)
+    if not alias:
+        np.testing.assert_array_equal(d_w.download(), w_now)
+    ftpa = oracle.split_transpose(taftp, nhg, A, A, 0, F)
+    ref = oracle.cb_tbtf(oracle.cb_power(ftpa, w_now, F, T, A, B, ts, 1), off, sc, T // ts)
+    np.testing.assert_array_equal(d_cb2.download(), ref)

@@ -127,8 +127,9 @@ int main(int argc, char** argv)
         for (void* p : inputs) fbf_host_free(p);
         const double block_ms = nsamples / channel_rate * 1e3;
         const double ops = 8.0 * config.nchans() * nsamples * 2 * config.cb_nantennas() * config.cb_nbeams();
-        std::printf("{\"what\": \"C++ Pipeline (Pipeline.cu:173-247 contract): pinned host block -> H2D, weights, "
-                    "split-transpose + IB, CB, D2H into the output rings, three streams\", "
+        std::printf("{\"what\": \"C++ Pipeline (Pipeline.cu:173-247 contract): pinned host block -> H2D; coherent "
+                    "and incoherent beams in one pass over TAFTP, the next block's weights in the reorder's "
+                    "launch; D2H into the output rings; three streams\", "
                     "\"geometry\": \"64 ant, 64 ch, 32 heap groups, 512 beams, ts 16, IB ts 16\", "
                     "\"input\": \"3 rotating pinned host blocks\", \"blocks\": %d, \"warmup\": %d, "
                     "\"ms_per_block\": %.4f, \"x_real_time\": %.3f, \"h2d_gbs\": %.2f, \"int8_tops\": %.2f, "

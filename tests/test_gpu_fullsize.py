@@ -3,12 +3,16 @@ against the oracle (CoherentBeamformerTester.cu:148-185 compares whole
 blocks the same way).
 
 * Config 2: one 64-antenna, 64-channel, 32-heap-group block (8192 spectra),
-  512 beams, tscrunch 16, with the benchmark's seed and launches: the
-  split-transpose + incoherent beam launch that also generates the block's
-  weights from the delay models (fbf_split_transpose_ib_weights), then the
-  coherent beamformer.  The beamformer's one-round grid partition depends
-  on F, T and the compute-unit count only, so this is the launch the
-  benchmark times.
+  512 beams, tscrunch 16, with the benchmark's seed.  The benchmark's
+  default step (test_config2_one_pass_blocks_bit_exact): both beams in one
+  pass over TAFTP, the reorder's launch generating the next block's
+  weights (fbf_cb_ib_beamform_taftp_next), over two blocks, the second
+  beamformed with the weights the first left.  The split-transpose path
+  (test_config2_block_bit_exact): the split-transpose + incoherent beam
+  launch that also generates the block's weights
+  (fbf_split_transpose_ib_weights), then the coherent beamformer.  The
+  coherent kernel's one-round grid partition depends on F, T and the
+  compute-unit count only, so these are the launches the benchmark times.
 * Config 3: the same block split 40/16/8 antennas with 768/256/64 beams plus
   the incoherent beam, as one captured graph after the autotuner, for
   tscrunch 4, 16 and 64; replayed on two blocks with eager work in between,
@@ -79,6 +83,55 @@ def test_config2_block_bit_exact(gpu, oracle, quant_mode):
     got = d_cb.download()
     np.testing.assert_array_equal(got, oracle.cb_tbtf(p, cb_off, cb_sc, cfg.cb_ohs(sub), quant_mode))
     assert got.astype(float).std() > 3.0 and np.mean(np.abs(got.astype(int)) == 127) < 0.01
+
+
+@pytest.mark.parametrize("quant_mode", [0, 1])
+def test_config2_one_pass_blocks_bit_exact(gpu, oracle, quant_mode):
+    capi = gpu
+    from psrdada_cpp_amd import workloads as wl
+    cfg = wl.config2()
+    sub = cfg.subarrays[0]
+    A, F, B, T, nhg = sub.nantennas, cfg.nchans, sub.nbeams, cfg.nsamples, cfg.nheap_groups
+    print(f"config 2 one-pass blocks: {A} ant, {F} ch, {T} spectra, {B} beams, ts {sub.tscrunch}, quant {quant_mode}",
+          flush=True)
+    (cb_off, cb_sc), (ib_off, ib_sc) = cfg.levels(sub)
+    s = capi.Stream()
+    delays = wl.synth_delays(B, A, BENCH_SEED + 1, cfg.tau_max_s)
+    d_delays = capi.DeviceBuffer.from_array(delays)
+    wp = capi.WeightsPlan(A, B, F, cfg.channel_frequencies())
+    cb = capi.CbPlan(A, B, F, sub.tscrunch, sub.fscrunch, cb_off, cb_sc, nsamples_per_heap_out=cfg.cb_ohs(sub),
+                     quant_mode=quant_mode)
+    ib = capi.IbPlan(cfg.total_nantennas, F, cfg.ib_tscrunch, cfg.ib_fscrunch, ib_off, ib_sc, quant_mode=quant_mode)
+    assert cb.ib_fused(ib, T)
+    d_w = capi.DeviceBuffer(cfg.weights_bytes(sub))
+    d_cb = capi.DeviceBuffer(cfg.cb_out_bytes(sub))
+    d_ib = capi.DeviceBuffer(cfg.ib_out_bytes())
+    epoch = cfg.block_seconds * 7
+    wp.generate(d_delays, epoch, d_w, stream=s)
+    for blk, seed in enumerate((BENCH_SEED, BENCH_SEED + 101)):
+        taftp = wl.synth_taftp(cfg, seed)
+        d_taftp = capi.DeviceBuffer.from_array(taftp)
+        s.sync()
+        w = d_w.download()   # this block's weights
+        w_ref = oracle.weights_f64(delays, cfg.channel_frequencies(), epoch, A, B, F)
+        dw = np.abs(w.astype(int) - w_ref.astype(int))
+        assert dw.max() <= 1 and (dw > 0).mean() < 1e-3, f"block {blk} weights"
+        d_cb.fill(0x5A, s)
+        d_ib.fill(0x5A, s)
+        cb.ib_beamform_taftp_next(ib, d_taftp, d_w, d_cb, d_ib, wp, d_delays, epoch + cfg.block_seconds, d_w,
+                                  stream=s)
+        s.sync()
+        ftpa = oracle.split_transpose(taftp, nhg, A, A, 0, F)
+        pib = oracle.ib_power(taftp, F, cfg.ib_tscrunch, cfg.ib_fscrunch, nhg, A, nthreads=16)
+        np.testing.assert_array_equal(d_ib.download(), oracle.ib_tf(pib, ib_off, ib_sc, quant_mode))
+        p = oracle.cb_power(ftpa, w, F, T, A, B, sub.tscrunch, sub.fscrunch, nthreads=16)
+        np.testing.assert_array_equal(d_cb.download(), oracle.cb_tbtf(p, cb_off, cb_sc, cfg.cb_ohs(sub), quant_mode))
+        d_taftp.free()
+        epoch += cfg.block_seconds
+    # The weights left for the next block equal the generator's at that epoch.
+    d_chk = capi.DeviceBuffer(cfg.weights_bytes(sub))
+    wp.generate(d_delays, epoch, d_chk)
+    np.testing.assert_array_equal(d_w.download(), d_chk.download())
 
 
 @pytest.mark.parametrize("ts", [4, 16, 64])

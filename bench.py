@@ -213,18 +213,26 @@ def main():
         if ev:
             ev[4].record(s)
 
-    def timed_blocks(K, k0, weights_every):
-        """K blocks back to back, events at block boundaries only."""
+    def timed_blocks(K, k0, weights_every, kernel_events=False):
+        """K blocks back to back, events at block boundaries (and, with
+        kernel_events, the plan's events right before and after the coherent
+        kernel of every block: fbf_cb_plan_set_timing)."""
         evs = [capi.Event() for _ in range(K + 1)]
+        kev = [(capi.Event(), capi.Event()) for _ in range(K)] if kernel_events else None
         capi.call("fbf_device_synchronize")
         t0 = time.perf_counter()
         evs[0].record(s)
         for k in range(K):
+            if kev:
+                cb.set_timing(*kev[k])
             step(k0 + k, weights_every=weights_every)
             evs[k + 1].record(s)
         s.sync()
         wall_ms = (time.perf_counter() - t0) * 1e3
-        return wall_ms, [evs[k].elapsed_ms(evs[k + 1]) for k in range(K)], evs[0].elapsed_ms(evs[K])
+        if kev:
+            cb.set_timing(None, None)
+        kern = [b.elapsed_ms(a) for b, a in kev] if kev else None
+        return wall_ms, [evs[k].elapsed_ms(evs[k + 1]) for k in range(K)], evs[0].elapsed_ms(evs[K]), kern
 
     # Clock settling: the GPU idles at a low clock and its power management
     # ramps up over tens of milliseconds of load, so untimed steps of the
@@ -245,12 +253,12 @@ def main():
 
     # Timed region: exactly K blocks, weights regenerated every block.
     K = args.steps
-    wall_ms, block_ms, dev_total_ms = timed_blocks(K, args.warmup, 1)
+    wall_ms, block_ms, dev_total_ms, kern_ms = timed_blocks(K, args.warmup, 1, kernel_events=True)
     ms_per_step = wall_ms / K
 
     # The same K blocks with the weights refreshed every 8th block (the
     # delay models change slowly; the serving engine refreshes on demand).
-    wall8_ms, _, _ = timed_blocks(K, args.warmup + K, 8)
+    wall8_ms, _, _, _ = timed_blocks(K, args.warmup + K, 8)
 
     # Per-kernel durations (separate pass, events around every kernel).
     R = min(50, max(10, K // 4))
@@ -284,7 +292,13 @@ def main():
     ops = cfg.cb_ops()
     block_s = cfg.block_seconds
     rt = block_s * 1e3 / ms_per_step
-    cb_tops = ops / (cb_ms * 1e-3) / 1e12
+    # The coherent kernel alone, from its events in the timed blocks; the
+    # window (the coherent kernel and what follows it in the call: the
+    # reorder, and on the one-pass path the IB finish and the next weights)
+    # from the per-kernel pass.
+    cb_kernel_ms = float(np.mean(kern_ms))
+    cb_tops = ops / (cb_kernel_ms * 1e-3) / 1e12
+    window_tops = ops / (cb_ms * 1e-3) / 1e12
     pipeline_tops = ops / (ms_per_step * 1e-3) / 1e12
     samples_per_s = (F * T * wl.NPOL * cfg.total_nantennas) / (ms_per_step * 1e-3)
     ib_bytes = cfg.taftp_bytes + cfg.ib_out_bytes()
@@ -344,6 +358,8 @@ def main():
         },
         "pipeline_int8_tops": round(pipeline_tops, 2),
         "cb_kernel_int8_tops": round(cb_tops, 2),
+        "cb_kernel_ms_timed_blocks": {"mean": round(cb_kernel_ms, 4), "median": round(float(np.median(kern_ms)), 4)},
+        "window_int8_tops": round(window_tops, 2),
         "samples_per_s": float(f"{samples_per_s:.4g}"),
         "block_latency_ms": {"p50": round(percentile(block_ms, 50), 4),
                              "p99": round(percentile(block_ms, 99), 4),
@@ -357,26 +373,31 @@ def main():
         "kernel_ms": kernel_ms,
         "kernel_hbm_gbs": kernel_gbs,
         "roofline": {
-            "kernel": ("coherent beamformer (cb_tr_k int8 MFMA + detect/quantise from TAFTP, with the incoherent "
-                       "beam's partials; then tbtf_k reorder + IB finish)" if fused_path else
-                       "coherent beamformer (cb_tr_k int8 MFMA + detect/quantise, then tbtf_k reorder)"),
+            "kernel": ("cb_tr_k: the coherent beamformer straight from TAFTP (int8 MFMA, exact detection, scrunch, "
+                       "quantise), leaving the incoherent beam's partials" if fused_path else
+                       "cb_tr_k: the coherent beamformer straight from TAFTP (int8 MFMA, exact detection, scrunch, "
+                       "quantise)" if taftp_path else
+                       "cb_tr_k: the coherent beamformer on FTPA (int8 MFMA, exact detection, scrunch, quantise)"),
             "bound": "mfma",
             "achieved": round(cb_tops, 2),
             "peak": wl.PEAK_INT8_TOPS,
             "unit": "TOP/s",
             "frac": round(cb_tops / wl.PEAK_INT8_TOPS, 4),
             "traffic": profiled_traffic(args.path),
-            "traffic_source": ("profile constant: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of the window's "
-                               "kernels on this workload, scaled by a plain copy's readings, from "
+            "traffic_source": ("profile constant: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of cb_tr_k on this "
+                               "workload, scaled by a plain copy's readings, from "
                                + os.path.relpath(TRAFFIC_PROFILES[args.path], ROOT) + " (not measured in this run)"
                                if args.path in TRAFFIC_PROFILES else "no committed profile for this path"),
             "ops_per_launch": ops,
-            "window": ("HIP events around fbf_cb_ib_beamform_taftp_next on its stream: cb_tr_k (staging the "
-                       "TAFTP block itself, with the IB partials), then tbtf_k's launch (the reorder, the IB finish "
-                       "and the next block's weights)" if fused_path
-                       else "HIP events around fbf_cb_beamform_taftp on its stream: cb_tr_k (staging the TAFTP "
-                       "block itself) plus the tbtf_k reorder" if taftp_path else
-                       "HIP events around fbf_cb_beamform on its stream: cb_tr_k plus the tbtf_k reorder"),
+            "timing": "HIP events recorded on the beamformer's stream right before and after cb_tr_k's launch in "
+                      "every timed block (fbf_cb_plan_set_timing); achieved = ops_per_launch / their mean",
+            "window": ("the call around cb_tr_k, from the per-kernel pass: fbf_cb_ib_beamform_taftp_next (cb_tr_k, "
+                       "then tbtf_k's launch: the reorder, the IB finish and the next block's weights)" if fused_path
+                       else "the call around cb_tr_k, from the per-kernel pass: fbf_cb_beamform_taftp (cb_tr_k, "
+                       "then the tbtf_k reorder)" if taftp_path else
+                       "the call around cb_tr_k, from the per-kernel pass: fbf_cb_beamform (cb_tr_k, then the "
+                       "tbtf_k reorder)"),
+            "window_achieved": round(window_tops, 2),
         },
     }
 
@@ -538,18 +559,16 @@ TRAFFIC_PROFILES = {
 
 
 def profiled_traffic(path):
-    """HBM bytes per launch of the window's kernels (coherent beamformer +
-    reorder, with the incoherent beam's partials and finish on the fused
-    path) on this workload, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary committed under profiles/ (counter readings scaled by what they
-    read for a plain 1 GiB copy); None when there is no summary for the
-    path."""
+    """HBM bytes per launch of cb_tr_k on this workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary committed under profiles/ (counter
+    readings scaled by what they read for a plain 1 GiB copy); None when
+    there is no summary for the path."""
     if path not in TRAFFIC_PROFILES:
         return None
     try:
         with open(TRAFFIC_PROFILES[path]) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith("kernel_traffic_bytes_per_launch:"):
                     return int(line.split(":", 1)[1])
     except OSError:
         pass

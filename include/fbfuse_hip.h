@@ -208,6 +208,12 @@ int fbf_cb_beamform_taftp(fbf_cb_plan* plan, const void* taftp, size_t taftp_byt
  * and one heap group under 2 GiB.  No counterpart in the reference. */
 int fbf_cb_taftp_direct(const fbf_cb_plan* plan, int nantennas_total, int nsamples_per_heap,
                         long long nsamples, int* direct);
+/* Timing of the coherent kernel alone: every later beamform call with this
+ * plan records `before` on its stream right before the coherent kernel's
+ * launch and `after` right after it (before the TBTF reorder and anything
+ * in that launch).  NULL, NULL stops it.  No counterpart in the reference;
+ * for benchmarks and profilers. */
+int fbf_cb_plan_set_timing(fbf_cb_plan* plan, fbf_event_t before, fbf_event_t after);
 /* LDS chunk length (columns of the integration window staged per step),
  * 0 = automatic; a power of two <= 16 dividing 2*tscrunch. */
 int fbf_cb_plan_set_chunk(fbf_cb_plan* plan, int chunk);

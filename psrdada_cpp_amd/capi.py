@@ -118,6 +118,7 @@ _SIGS = {
     "fbf_cb_ib_beamform_taftp_next": [vp, vp, vp, sz, i32, vp, sz, vp, sz, vp, sz, vp, vp, sz, C.c_double, vp, sz,
                                       vp],
     "fbf_cb_plan_set_chunk": [vp, i32],
+    "fbf_cb_plan_set_timing": [vp, vp, vp],
     "fbf_cb_autotune": [vp, C.c_longlong, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)],
     "fbf_batch_create": [i32, i32, i32, i32, i32, C.POINTER(SubarrayDesc), C.POINTER(C.c_void_p),
                          C.POINTER(C.c_void_p), pvp],
@@ -425,6 +426,12 @@ class CbPlan:
     def beamform(self, ftpa, weights, out, stream=None):
         call("fbf_cb_beamform", self.handle, ftpa.ptr, ftpa.nbytes, weights.ptr, weights.nbytes,
              out.ptr, out.nbytes, _s(stream))
+
+    def set_timing(self, before=None, after=None):
+        """Events recorded around the coherent kernel alone in later calls
+        (None, None: stop)."""
+        call("fbf_cb_plan_set_timing", self.handle, before.handle if before else None,
+             after.handle if after else None)
 
     def set_chunk(self, c):
         call("fbf_cb_plan_set_chunk", self.handle, int(c))

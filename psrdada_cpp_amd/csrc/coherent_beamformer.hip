@@ -2364,6 +2364,7 @@ struct fbf_cb_plan {
     size_t ftpa_bytes;
     void* ibp;        // incoherent-beam partials of fbf_cb_ib_beamform_taftp
     size_t ibp_bytes;
+    hipEvent_t t_before, t_after;   // recorded around the coherent kernel (fbf_cb_plan_set_timing)
 };
 
 static long long* cb_thr(const fbf_cb_plan* plan)
@@ -2437,7 +2438,7 @@ extern "C" int fbf_cb_plan_create(const fbf_cb_desc* desc, const float* offsets,
     const int Fo = desc->nchans / desc->fscrunch;
     int ohs = desc->nsamples_per_heap_out ? desc->nsamples_per_heap_out : 8192 / Fo;
     FBF_REQUIRE(ohs > 0, FBF_ERR_ARG, "fbf_cb_plan_create: output heap holds no samples");
-    auto* p = new fbf_cb_plan{*desc, ohs, 0, 0, nullptr, nullptr, 0, nullptr, 0, nullptr, 0};
+    auto* p = new fbf_cb_plan{*desc, ohs, 0, 0, nullptr, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, nullptr};
     hipError_t e = hipMalloc(&p->d_levels, sizeof(float) * 2 * Fo + sizeof(long long) * kThr * Fo);
     if (e != hipSuccess) {
         delete p;
@@ -2659,6 +2660,8 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
     }
 
     const CbVariant var = cb_pick_variant(d, tv ? tv->variant : plan->variant, mode, T);
+    // The timing events (fbf_cb_plan_set_timing), around the coherent kernel only.
+    auto mark = [&](hipEvent_t ev) { return ev ? hipEventRecord(ev, s) : hipSuccess; };
     FBF_REQUIRE(!tv || (var.tr && !var.ws && var.rows != 2 && d.fscrunch == 1), FBF_ERR_STATE,
                 "fbf_cb: the TAFTP input needs the transposed kernel with 64 or 128 beams per wave, fscrunch 1");
     if (var.ws) {
@@ -2689,7 +2692,9 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
         wa.ntiles = (int)(2 * T / 32);
         wa.nbg = (d.nbeams + 63) / 64;
         const int ksteps = (2 * d.nantennas + 31) / 32;
-        hipError_t e = launch_ws(wa, W, ksteps, s);
+        hipError_t e = mark(plan->t_before);
+        if (e == hipSuccess) e = launch_ws(wa, W, ksteps, s);
+        if (e == hipSuccess) e = mark(plan->t_after);
         if (e != hipSuccess) return hip_status(e, "cb_ws_k launch");
         e = launch_tbtf(tbtf_args(static_cast<const int8_t*>(kout), static_cast<int8_t*>(out), To, d.nbeams, Fo,
                                   plan->ohs),
@@ -2737,9 +2742,12 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
         const int beams_per_wg = 16 * rt * (4 / d.fscrunch);
         ta.n_bgroups = (d.nbeams + beams_per_wg - 1) / beams_per_wg;
         const int ksteps = (2 * d.nantennas + 31) / 32;
-        hipError_t e = rt == 2   ? launch_tr<2>(ta, d.fscrunch, W, ksteps, passes, s)
-                       : rt == 4 ? launch_tr<4>(ta, d.fscrunch, W, ksteps, passes, s)
-                                 : launch_tr<8>(ta, d.fscrunch, W, ksteps, passes, s);
+        hipError_t e = mark(plan->t_before);
+        if (e == hipSuccess)
+            e = rt == 2   ? launch_tr<2>(ta, d.fscrunch, W, ksteps, passes, s)
+                : rt == 4 ? launch_tr<4>(ta, d.fscrunch, W, ksteps, passes, s)
+                          : launch_tr<8>(ta, d.fscrunch, W, ksteps, passes, s);
+        if (e == hipSuccess) e = mark(plan->t_after);
         if (e != hipSuccess) return hip_status(e, "cb_tr_k launch");
         e = launch_tbtf(tbtf_args(static_cast<const int8_t*>(kout), static_cast<int8_t*>(out), To, d.nbeams, Fo,
                                   plan->ohs),
@@ -2795,7 +2803,8 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
     const long long nblocks = (long long)Fo * a.n_tgroups * a.n_bgroups;
     FBF_REQUIRE(nblocks > 0 && nblocks < (1ll << 31), FBF_ERR_UNSUPPORTED, "fbf_cb: grid too large");
     dim3 grid((unsigned)nblocks);
-    hipError_t e;
+    hipError_t e = mark(plan->t_before);
+    if (e != hipSuccess) return hip_status(e, "hipEventRecord");
     if (var.lds) {
         if (mode == MODE_INT8)
             e = launch_lds_mode<MODE_INT8>(a, var.rows, ksteps, jc, grid, s);
@@ -2818,6 +2827,7 @@ static int cb_launch(fbf_cb_plan* plan, int mode, const void* ftpa, long long T,
         else
             e = launch_mode<MODE_POWER>(a, var.rows, ksteps, grid, s);
     }
+    if (e == hipSuccess) e = mark(plan->t_after);
     if (e != hipSuccess) return hip_status(e, "cb_mfma_k launch");
     if (mode == MODE_INT8) {
         e = launch_tbtf(tbtf_args(static_cast<const int8_t*>(kout), static_cast<int8_t*>(out), To, d.nbeams, Fo,
@@ -3046,6 +3056,16 @@ extern "C" int fbf_cb_taftp_direct(const fbf_cb_plan* plan, int nantennas_total,
     FBF_REQUIRE(plan && direct, FBF_ERR_ARG, "fbf_cb_taftp_direct: null argument");
     *direct = nsamples > 0 && nsamples % plan->d.tscrunch == 0 &&
               cb_taftp_direct(plan, nsamples, nantennas_total, nsamples_per_heap);
+    return FBF_OK;
+}
+
+extern "C" int fbf_cb_plan_set_timing(fbf_cb_plan* plan, fbf_event_t before, fbf_event_t after)
+{
+    FBF_REQUIRE(plan, FBF_ERR_ARG, "fbf_cb_plan_set_timing: null plan");
+    FBF_REQUIRE((before == nullptr) == (after == nullptr), FBF_ERR_ARG,
+                "fbf_cb_plan_set_timing: give both events or neither");
+    plan->t_before = static_cast<hipEvent_t>(before);
+    plan->t_after = static_cast<hipEvent_t>(after);
     return FBF_OK;
 }
 

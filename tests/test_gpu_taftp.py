@@ -297,3 +297,34 @@ def test_cb_ib_next_weights(gpu, oracle, A, B, variant, alias):
     ftpa = oracle.split_transpose(taftp, nhg, A, A, 0, F)
     ref = oracle.cb_tbtf(oracle.cb_power(ftpa, w_now, F, T, A, B, ts, 1), off, sc, T // ts)
     np.testing.assert_array_equal(d_cb2.download(), ref)
+
+
+def test_cb_plan_timing_events(gpu, oracle):
+    """fbf_cb_plan_set_timing: the plan records the events around the
+    coherent kernel alone, inside the call's own span, on every later call
+    until unset; a half-set pair is refused."""
+    capi = gpu
+    A, F, nhg, B, ts, S = 64, 8, 2, 512, 16, 256
+    T = nhg * S
+    off, sc = _levels(oracle, A, ts, 1, F)
+    plan = capi.CbPlan(A, B, F, ts, 1, off, sc, nsamples_per_heap_out=T // ts)
+    _, _, ibo, ibs = oracle.power_levels(16.0, 24.0, A, ts, 1, A, 16, 1)
+    ib = capi.IbPlan(A, F, 16, 1, np.full(F, ibo, np.float32), np.full(F, ibs, np.float32))
+    d_t = _dev(capi, gaussian_int8(2900, nhg * A * F * S * 4, 16.0))
+    d_w = _dev(capi, unit_weights(2901, F * B * A))
+    d_cb, d_tf = capi.DeviceBuffer(F * (T // ts) * B), capi.DeviceBuffer(ib.output_bytes(d_t.nbytes))
+    s = capi.Stream()
+    c0, c1, k0, k1 = capi.Event(), capi.Event(), capi.Event(), capi.Event()
+    plan.set_timing(k0, k1)
+    c0.record(s)
+    plan.ib_beamform_taftp(ib, d_t, d_w, d_cb, d_tf, stream=s)
+    c1.record(s)
+    s.sync()
+    kern, call, before = k0.elapsed_ms(k1), c0.elapsed_ms(c1), c0.elapsed_ms(k0)
+    assert 0 < kern <= call and 0 <= before <= call - kern
+    plan.set_timing(None, None)
+    plan.ib_beamform_taftp(ib, d_t, d_w, d_cb, d_tf, stream=s)
+    s.sync()
+    assert k0.elapsed_ms(k1) == kern   # not recorded again
+    with pytest.raises(capi.FbfError):
+        capi.call("fbf_cb_plan_set_timing", plan.handle, k0.handle, None)

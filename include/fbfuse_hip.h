@@ -194,7 +194,9 @@ int fbf_cb_beamform_subarray(fbf_cb_plan* plan, const void* ftpa_block, size_t b
  * Where fbf_cb_taftp_direct says so, the kernel stages the block's
  * [antenna][sample, pol] runs itself and transposes them in LDS, so no FTPA
  * copy is written; otherwise the call split-transposes into an FTPA buffer
- * that the plan owns (grown on demand, as the staging buffer) first.  With
+ * that the plan owns (grown on demand, as the staging buffer) first.  As
+ * with fbf_cb_beamform, calls with one plan on different streams must not
+ * overlap (they share the plan's buffers).  With
  * the automatic variant and more than 48 antennas it picks the transposed
  * kernel with 128 beams per wave where that applies (half the transposed
  * LDS reads per MFMA of 64 beams per wave). */
@@ -262,8 +264,10 @@ int fbf_ib_power(fbf_ib_plan* plan, const void* taftp, size_t taftp_bytes,
  * reference Pipeline's three operator calls (Pipeline.cu:166-170).  Where
  * fbf_cb_ib_fused says so, the coherent kernel also leaves each sample's
  * power over the antennas and pols (from the operands it already holds)
- * in a buffer the coherent plan owns, and a small kernel scrunches and
- * quantises them; otherwise the two beamformers run one after the other. */
+ * in a buffer the coherent plan owns, and the first work-groups of the
+ * TBTF reorder's launch scrunch and quantise them; otherwise the two
+ * beamformers run one after the other.  Calls with one plan on different
+ * streams must not overlap (they share the plan's buffers). */
 int fbf_cb_ib_beamform_taftp(fbf_cb_plan* plan, fbf_ib_plan* ib_plan, const void* taftp, size_t taftp_bytes,
                              int nsamples_per_heap, const void* fba_weights, size_t weights_bytes,
                              void* tbtf, size_t tbtf_bytes, void* tf, size_t tf_bytes, fbf_stream_t stream);

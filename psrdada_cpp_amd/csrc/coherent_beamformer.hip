@@ -2983,6 +2983,8 @@ static int cb_ib_run(fbf_cb_plan* plan, fbf_ib_plan* ib_plan, const void* taftp,
         if (st) return st;
         FBF_REQUIRE(next->weights_bytes == weights_bytes, FBF_ERR_SIZE,
                     "fbf_cb_ib_beamform_taftp_next: the next weights differ in size from the block's");
+        FBF_REQUIRE(wj.w.F == d.nchans && (long long)wj.w.nd == (long long)d.nantennas * d.nbeams, FBF_ERR_ARG,
+                    "fbf_cb_ib_beamform_taftp_next: the weights plan's geometry differs from the coherent plan's");
         FBF_REQUIRE((unsigned long long)wj.w.gx * wj.w.gy < (1ull << 30), FBF_ERR_UNSUPPORTED,
                     "fbf_cb_ib_beamform_taftp_next: too many weights work-groups");
         wj.nblocks = (int)(wj.w.gx * wj.w.gy);

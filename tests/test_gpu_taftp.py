@@ -328,3 +328,23 @@ def test_cb_plan_timing_events(gpu, oracle):
     assert k0.elapsed_ms(k1) == kern   # not recorded again
     with pytest.raises(capi.FbfError):
         capi.call("fbf_cb_plan_set_timing", plan.handle, k0.handle, None)
+
+
+def test_cb_ib_next_weights_rejects_other_geometry(gpu, oracle):
+    """A weights plan of other channels with the same byte count (half the
+    channels, twice the beams) is refused before anything is launched.  (A
+    plan of another antennas x beams split with the same product gives the
+    same bytes: the generator works per (beam, antenna) index.)"""
+    capi = gpu
+    A, B, F, ts, S = 64, 256, 4, 16, 256
+    off, sc = _levels(oracle, A, ts, 1, F)
+    plan = capi.CbPlan(A, B, F, ts, 1, off, sc, nsamples_per_heap_out=S // ts)
+    ib = capi.IbPlan(A, F, 16, 1, np.zeros(F, np.float32), np.ones(F, np.float32))
+    freqs = capi.channel_frequencies_f64(1284e6, 856e6, F)
+    wp_other = capi.WeightsPlan(A, 2 * B, F // 2, freqs[: F // 2])   # same bytes, other channels
+    d_t = capi.DeviceBuffer(A * F * S * 4)
+    d_w = capi.DeviceBuffer(F * B * A * 2)
+    d_d = capi.DeviceBuffer(2 * B * A * 8)
+    d_cb, d_tf = capi.DeviceBuffer(F * (S // ts) * B), capi.DeviceBuffer(ib.output_bytes(d_t.nbytes))
+    with pytest.raises(capi.FbfError):
+        plan.ib_beamform_taftp_next(ib, d_t, d_w, d_cb, d_tf, wp_other, d_d, 1.0, d_w)

@@ -20,9 +20,9 @@
 // as FTPA stores a column (char2 per antenna), so every 16-byte K slice of a
 // voltage column is one contiguous 16-byte load straight into the MFMA B
 // operand -- no de-interleave and no LDS staging.  The weights are embedded in
-// registers from the raw FBA char2 bytes (byte swap for Im rows, saturating
-// negation of the odd bytes for Re rows), once per channel per wave; waves
-// whose weights hold a wi = -128, where the negation saturates, take an exact
+// registers from the raw FBA char2 bytes (byte swap for Im rows, negation
+// of the odd bytes for Re rows), once per channel per wave; waves whose
+// weights hold a wi = -128, whose negation int8 cannot hold, take an exact
 // path instead (cb_exact_outputs), so the sums are exact for every int8
 // weight.
 //
@@ -113,40 +113,42 @@ __device__ __forceinline__ v4i load_wslice(const int8_t* col, int k0, int K, boo
     return v4i{lo_ok ? r.x : 0, lo_ok ? r.y : 0, hi_ok ? r.z : 0, hi_ok ? r.w : 0};
 }
 
-// Im row: [wr, wi] -> [wi, wr] per antenna (swap bytes in each 16-bit half).
+// Im row: [wr, wi] -> [wi, wr] per antenna (swap the bytes of each 16-bit
+// half: one byte permutation).
 __device__ __forceinline__ uint32_t swap_pairs(uint32_t x)
 {
-    return ((x >> 8) & 0x00FF00FFu) | ((x << 8) & 0xFF00FF00u);
+    return __builtin_amdgcn_perm(x, x, 0x02030001u);
 }
 
-__device__ __forceinline__ uint32_t sat_neg_byte(uint32_t b)
-{
-    const int v = (int)(int8_t)(b & 0xFF);
-    return (uint32_t)(v == -128 ? 127 : -v) & 0xFFu;
-}
+typedef unsigned short us2_t __attribute__((ext_vector_type(2)));
 
-// Re row: [wr, wi] -> [wr, -wi] per antenna, negation saturating at 127.
+// Re row: [wr, wi] -> [wr, -wi] per antenna.  The odd byte of each 16-bit
+// half is negated by a packed 16-bit subtract from zero: its low byte is
+// zero, so no borrow reaches the even byte.  -wi is exact for wi != -128
+// (wi = -128 gives -128 again); weights holding -128 never reach the
+// embedded form (below).
 __device__ __forceinline__ uint32_t neg_odd(uint32_t x)
 {
-    return (x & 0x00FF00FFu) | (sat_neg_byte(x >> 8) << 8) | (sat_neg_byte(x >> 24) << 24);
+    const us2_t n = us2_t{0, 0} - __builtin_bit_cast(us2_t, x & 0xFF00FF00u);
+    return (x & 0x00FF00FFu) | __builtin_bit_cast(uint32_t, n);
 }
 
 // Embedded 16-byte weight slice of an Im row (swap_pairs) or a Re row
-// (neg_odd), without branches: both forms are computed and a bit mask picks.
+// (neg_odd).  The callers pass im_row as a constant, so only one form is
+// compiled.
 __device__ __forceinline__ v4i embed_slice(v4i raw, bool im_row)
 {
-    const uint32_t m = im_row ? 0xFFFFFFFFu : 0u;
     v4i e;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const uint32_t x = (uint32_t)raw[q];
-        e[q] = (int)((swap_pairs(x) & m) | (neg_odd(x) & ~m));
+        e[q] = (int)(im_row ? swap_pairs(x) : neg_odd(x));
     }
     return e;
 }
 
-// Weight components wi = -128.  The Re row carries sat(-wi), which is 127
-// for wi = -128, one short of -wi = 128, while the reference's int32 sums
+// Weight components wi = -128.  The Re row would carry -wi = 128, which
+// int8 cannot hold, while the reference's int32 sums
 // (r = xx - yy, CoherentBeamformer.cu:150-155) are exact for every int8
 // weight.  Weights round(127 e^{i theta}) never hold -128, so the MFMA paths
 // stay as they are and every kernel first checks the weights it will use:
@@ -1363,15 +1365,38 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     }
     const size_t tile_stride = (size_t)32 * p.vstride;
     const int tmask = (1 << ta.lgtph) - 1;
+    // TAFTP: each load keeps a running 32-bit offset of its next tile (the
+    // block is below 4 GiB, fbf_cb_beamform_taftp's condition), since the
+    // issues take consecutive tiles: 64 bytes on within a heap, the next
+    // heap's first tile at a heap boundary; loads past the last tile keep
+    // re-reading it.  A handful of scalar instructions per load instead of a
+    // 64-bit tile address (every wave of a compute unit shares one scalar
+    // unit, and the loads are issued right after a barrier).
+    uint32_t toff_run[St::M];
+    int tile_run[St::M];
+    if constexpr (TA) {
+#pragma unroll
+        for (int m = 0; m < St::M; ++m) {
+            const int t = gtile[m] < ta.ntiles ? gtile[m] : ta.ntiles - 1;
+            tile_run[m] = gtile[m];
+            toff_run[m] = (uint32_t)(t >> ta.lgtph) * (uint32_t)ta.hstride + ((uint32_t)(t & tmask) << 6);
+            cbase[m] = uniform_ptr(cbase[m]);
+        }
+    }
+    const uint32_t heap_jump = (uint32_t)ta.hstride - (uint32_t)tmask * 64u;
     auto issue = [&](int q, int slot) {
         int8_t* set = lds + slot * St::SET;
 #pragma unroll
         for (int m = 0; m < St::M; ++m) {
-            const int tt = gtile[m] + q < ta.ntiles ? gtile[m] + q : ta.ntiles - 1;
             int8_t* dst = ldst[m] >= 0 ? set + ldst[m] : dummy + wv * 1024;
-            const size_t toff =
-                TA ? (size_t)(tt >> ta.lgtph) * ta.hstride + (size_t)(tt & tmask) * 64 : (size_t)tt * tile_stride;
-            dma16(uniform_ptr(cbase[m] + toff) + goff[m], dst);
+            if constexpr (TA) {
+                dma16(cbase[m] + toff_run[m] + goff[m], dst);
+                ++tile_run[m];
+                if (tile_run[m] < ta.ntiles) toff_run[m] += (tile_run[m] & tmask) ? 64u : heap_jump;
+            } else {
+                const int tt = gtile[m] + q < ta.ntiles ? gtile[m] + q : ta.ntiles - 1;
+                dma16(uniform_ptr(cbase[m] + (size_t)tt * tile_stride) + goff[m], dst);
+            }
         }
     };
     // Physical offsets of this lane's K pieces in its column (FTPA), or of
@@ -2888,7 +2913,8 @@ static bool cb_taftp_direct(const fbf_cb_plan* plan, long long T, int nantennas_
     const fbf_cb_desc& d = plan->d;
     const CbVariant var = cb_pick_variant(d, cb_taftp_variant(plan, T), MODE_INT8, T);
     return var.tr && !var.ws && var.rows != 2 && d.fscrunch == 1 && S >= 16 && (S & (S - 1)) == 0 &&
-           (long long)4 * nantennas_total * d.nchans * S < (1ll << 31);
+           (long long)4 * nantennas_total * d.nchans * S < (1ll << 31) &&
+           (long long)4 * nantennas_total * d.nchans * T < (1ll << 32);   // 32-bit tile offsets
 }
 
 extern "C" int fbf_cb_beamform_taftp(fbf_cb_plan* plan, const void* taftp, size_t taftp_bytes, int nantennas_total,

@@ -93,6 +93,20 @@ def test_cb_taftp_minus128_weights_exact(gpu, oracle, variant, Atot, A, a0):
     _check_taftp(gpu, oracle, Atot, A, a0, F, nhg, B, ts, fs, S, variant, True, 2200, weights=w.reshape(-1))
 
 
+def test_cb_taftp_direct_needs_block_below_4gib(gpu, oracle):
+    """The direct path addresses the block with 32-bit tile offsets, so a
+    block of 4 GiB or more takes the split-transpose path (a query only: no
+    such block is allocated)."""
+    capi = gpu
+    A, F, B, ts, S = 64, 64, 512, 16, 256
+    off, sc = _levels(oracle, A, ts, 1, F)
+    plan = capi.CbPlan(A, B, F, ts, 1, off, sc)
+    hg = 4 * A * F * S                    # bytes per heap group
+    below = ((1 << 32) - 1) // hg          # heap groups of the largest block below 4 GiB
+    assert plan.taftp_direct(A, below * S, S)
+    assert not plan.taftp_direct(A, (below + 1) * S, S)
+
+
 def test_cb_taftp_matches_ftpa_path_and_repeats(gpu, oracle):
     """The direct path equals split-transpose + fbf_cb_beamform on the GPU
     at a larger block (64 channels, 4 heap groups, 512 beams), on two calls

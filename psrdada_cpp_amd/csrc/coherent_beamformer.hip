@@ -1486,6 +1486,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     // Output windows number below 2^31 (cb_tr_applies), so window indices
     // compare as int: a scalar compare for the wave-uniform rows.
     const int nout = (int)p.nsamples_out;
+    const bool all_beams = wave_active && beam_base + 16 * RT <= p.B;
     // All of a pass's values are estimated first, then their thresholds
     // read (the LDS reads in flight together, one wait), then stored.
     auto store_pass = [&](const long long (&t)[VALS], int w0) {
@@ -1505,6 +1506,19 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 #pragma unroll
             for (int v = 0; v < VALS; ++v) q[v] = quantise((double)t[v], q_off, q_sc, p.quant);
         }
+        // Every beam of the wave and every window of the pass in range (the
+        // usual case, uniform over the wave): the four rows' bases are
+        // scalar and the stores need no per-store checks.
+        if constexpr (RT != 2 && FS == 1) {
+            if (all_beams && w0 + NSLOT <= nout) {
+                int8_t* const base = out_f + (size_t)w0 * p.B;
+#pragma unroll
+                for (int v = 0; v < VALS; ++v)
+                    global_ptr(uniform_ptr(base + (size_t)(v % NSLOT) * p.B + 64 * (v / NSLOT)))[(unsigned)lane_off] =
+                        q[v];
+                return;
+            }
+        }
 #pragma unroll
         for (int v = 0; v < VALS; ++v) {
             if (FS > 1 && v % FS != wc) continue;
@@ -1516,6 +1530,9 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
                 global_ptr(uniform_ptr(out_f + (size_t)row * p.B + bsh))[(unsigned)lane_off] = q[v];
         }
     };
+    // First window of the pass; tile0 is a multiple of the slot's tiles, so
+    // the division is exact.
+    int w0_pass = tile0 * SPT / ta.tps;
     for (int pass = 0; pass < ta.npasses; ++pass) {
         long long pw[NG][NSLOT];
 #pragma unroll
@@ -1608,7 +1625,8 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
             block_barrier();
         }
 
-        store_pass(tot, (tile0 + pass * ta.ntp) * SPT / ta.tps);
+        store_pass(tot, w0_pass);
+        w0_pass += NSLOT;   // a pass holds NSLOT windows (ta.ntp = NSLOT / SPT * tps tiles)
     }
     // The loads past the last tile are still in flight: LDS must not be
     // released before they land.

@@ -1249,11 +1249,13 @@ struct TrRed {
 //
 // A work-group has 4 * NTS waves: NTS time slots (contiguous runs of passes
 // of the same channel group and beams) x 4 waves ((4 / FS) beam groups x FS
-// channels).  A work-group holds every wave of its compute unit (NTS =
-// tr_nts waves per SIMD: 2, 3 or 4), so the
-// per-step barrier keeps them in lockstep: waves of separate work-groups
-// sharing a SIMD progress at different rates (the issue arbiter prefers
-// older waves), which leaves SIMDs with one wave at the end.
+// channels).  With 64 beams per wave a work-group holds every wave of its
+// compute unit (NTS = tr_nts waves per SIMD: 3 or 4), so the per-step
+// barrier keeps them in lockstep: waves of separate work-groups sharing a
+// SIMD progress at different rates (the issue arbiter prefers older waves),
+// which leaves SIMDs with one wave at the end.  With 128 beams per wave
+// (two waves per SIMD) a work-group is one time slot, two share a compute
+// unit, and they take turns at a raised wave priority instead.
 template <int RT, int KSTEPS, int SPT, int NSLOT, int FS, int NTS, bool TA, bool IBF>
 __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 {
@@ -1266,7 +1268,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     static_assert(RT == 2 || RT == 4 || RT == 8, "RT");
     static_assert(NSLOT % SPT == 0 && (RT != 2 || NSLOT % 2 == 0), "slots");
     static_assert(St::M <= 2, "at most two staging loads per wave per set");
-    static_assert(!IBF || (TA && FS == 1 && KSTEPS <= 4 && NTS == 2), "incoherent-beam partials");
+    static_assert(!IBF || (TA && FS == 1 && KSTEPS <= 4 && NTS <= 2), "incoherent-beam partials");
     extern __shared__ __align__(16) int8_t lds[];
     const CbArgs& p = ta.p;
 
@@ -1297,8 +1299,11 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
 
     // Start-up loads first, all in flight together: the channel's quantiser
     // thresholds (to LDS below), the weights, then the first staged tiles.
-    static_assert(256 * NTS >= kThr, "one threshold per thread");
+    // One threshold per thread, two for the first threads of a one-slot
+    // work-group (256 threads, kThr = 257).
+    static_assert(512 * NTS >= kThr, "at most two thresholds per thread");
     const long long thr_v = tid < kThr ? p.thr[(size_t)fo * kThr + tid] : 0;
+    const long long thr_v2 = tid + 256 * NTS < kThr ? p.thr[(size_t)fo * kThr + tid + 256 * NTS] : 0;
 
     // Weights (B operand).  Tile r holds the Re (r even) or Im (r odd)
     // embedding rows of the 32 beams beam_base + 32 (r / 2) + n, n = lane %
@@ -1472,6 +1477,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     const bool q_f32 = q_thr && thr_f32_applies(q_off, q_sc);
     const QuantConst qz = quant_const(q_off, q_sc);
     if (tid < kThr) thr[tid] = thr_v;
+    if (tid + 256 * NTS < kThr) thr[tid + 256 * NTS] = thr_v2;
     const long long* const thr_mid = thr + 128;
     // Quantise and store one pass's totals (staged [fo][to][b] int8); with
     // FS > 1 the waves of a beam group share the totals out.
@@ -1530,6 +1536,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
                 global_ptr(uniform_ptr(out_f + (size_t)row * p.B + bsh))[(unsigned)lane_off] = q[v];
         }
     };
+    const int prio_phase = blockIdx.x >= gridDim.x / 2 ? 1 : 0;
     // First window of the pass; tile0 is a multiple of the slot's tiles, so
     // the division is exact.
     int w0_pass = tile0 * SPT / ta.tps;
@@ -1562,6 +1569,18 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
                 // done once WAIT later loads remain; the barrier makes every
                 // wave's loads visible and frees the G slots read since the
                 // last barrier, which are refilled NB - G steps ahead.
+                // One-slot work-groups (two per compute unit, one wave each
+                // per SIMD): the issue arbiter favours the older wave of a
+                // SIMD, so the two work-groups take turns at a raised
+                // priority, every 16 steps in opposite phase (the halves of
+                // the grid), instead of the older one finishing first and
+                // leaving its SIMDs one wave for the rest.
+                if constexpr (NTS == 1) {
+                    if (step % 16 == 0) {
+                        if (((step >> 4) + prio_phase) & 1) __builtin_amdgcn_s_setprio(2);
+                        else __builtin_amdgcn_s_setprio(0);
+                    }
+                }
                 if (step % St::G == 0) {
                     wait_vm_lgkm0<St::WAIT>();
                     block_barrier();
@@ -1650,12 +1669,16 @@ inline int cb_tr_nslot(int rt, int W)
 
 inline int cb_tr_nb(int ni) { return 60 / ni < 2 ? 2 : (60 / ni > 8 ? 8 : 60 / ni); }
 
-// Time slots per work-group of the transposed kernel (waves per SIMD): four
-// with 64 beams per wave and two windows per pass (at most 128 VGPRs a
-// wave), two with 128 beams per wave (256 VGPRs), else three.  With four
-// waves a SIMD overlaps one wave's MFMAs with the others' detection better:
-// at 64 antennas the VALU work per step outweighs the MFMA work.
-constexpr int tr_nts(int rt, int nslot) { return rt == 8 ? 2 : (rt == 4 && nslot == 2 ? 4 : 3); }
+// Time slots per work-group of the transposed kernel: four with 64 beams
+// per wave and two windows per pass (at most 128 VGPRs a wave, four waves per
+// SIMD), one with 128 beams per wave (256 VGPRs, two work-groups and so two
+// waves per SIMD), else three.  With four waves a SIMD overlaps one wave's
+// MFMAs with the others' detection better: at 64 antennas the VALU work per
+// step outweighs the MFMA work.  With 128 beams per wave, one work-group of
+// two time slots kept its two waves of a SIMD in lockstep, and the older one
+// waited at every barrier for the younger; two one-slot work-groups with
+// alternating priority measured 2.0-2.4 % faster.
+constexpr int tr_nts(int rt, int nslot) { return rt == 8 ? 1 : (rt == 4 && nslot == 2 ? 4 : 3); }
 
 inline size_t cb_tr_shm(int fs, int ksteps, int vals, int nts)
 {

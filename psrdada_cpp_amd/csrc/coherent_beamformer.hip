@@ -1120,6 +1120,8 @@ __device__ __forceinline__ QuantConst quant_const(float off, float sc)
 // by less than 1, and then |y - Y| < 1 suffices), so |e - q| < 2 in either
 // mode (the clamp keeps this), i.e. e is q or a neighbour.
 constexpr int kThr = 257;
+constexpr int kThrBytes = 2304;   // kThr int64, padded to a multiple of 256 bytes
+static_assert(kThrBytes >= kThr * 8, "threshold table");
 
 __global__ __launch_bounds__(256) void cb_thresholds_k(const float* offsets, const float* scales, int mode,
                                                        long long* thr)
@@ -1327,9 +1329,13 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     // antenna rows of 64 bytes (its 32 columns, contiguous in TAFTP), and a
     // load fills 16 rows, 4 lanes a row; rows past A re-read antenna 0.
     const int kpieces = K / 16;
-    int8_t* dummy = lds + NB * St::SET;
+    // LDS: the channel's thresholds first (at address 0, so that a
+    // threshold read takes the table's offset as its immediate), then the
+    // ring, the dummy run and the fscrunch reduction.
+    long long* thr = reinterpret_cast<long long*>(lds);                   // [kThr] of channel fo
+    int8_t* const ring = lds + kThrBytes;
+    int8_t* dummy = ring + NB * St::SET;
     long long* red = reinterpret_cast<long long*>(dummy + St::DUMMY);   // [waves][VALS][64]
-    long long* thr = red + (FS > 1 ? St::NW * VALS * 64 : 0);           // [kThr] of channel fo
     // A set holds one tile per (time slot, channel): tile index
     // tsl * FS + ch.  Load m of this wave fills 1 KiB of tile ti = u / KSTEPS
     // (u = wave + NW * m, the same tile for the whole wave): the channel's
@@ -1390,7 +1396,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     }
     const uint32_t heap_jump = (uint32_t)ta.hstride - (uint32_t)tmask * 64u;
     auto issue = [&](int q, int slot) {
-        int8_t* set = lds + slot * St::SET;
+        int8_t* set = ring + slot * St::SET;
 #pragma unroll
         for (int m = 0; m < St::M; ++m) {
             int8_t* dst = ldst[m] >= 0 ? set + ldst[m] : dummy + wv * 1024;
@@ -1496,7 +1502,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
     // All of a pass's values are estimated first, then their thresholds
     // read (the LDS reads in flight together, one wait), then stored.
     auto store_pass = [&](const long long (&t)[VALS], int w0) {
-        int8_t q[VALS];
+        int q[VALS];   // (ints: no packing into dwords before the byte stores)
         if (q_thr) {
             int e[VALS];
             if (q_f32) {
@@ -1592,7 +1598,7 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
                 }
                 const bool ib_now = ib_live && ((step + 2 * tsl) & 3) == (wv & 3);
                 if (live || ib_now) {
-                    const int8_t* src = lds + rs * St::SET;
+                    const int8_t* src = ring + rs * St::SET;
                     v4i a[KSTEPS];
                     if constexpr (TA) {
                         tr_operands<KSTEPS>(src + aoff[0], a);
@@ -1685,7 +1691,7 @@ inline size_t cb_tr_shm(int fs, int ksteps, int vals, int nts)
     const int nw = 4 * nts;
     const int ni = nts * fs * ksteps;
     return (size_t)cb_tr_nb(ni) * ni * 1024 + ((ni % nw) ? (size_t)nw * 1024 : 0) +
-           (fs > 1 ? (size_t)nw * vals * 64 * sizeof(long long) : 0) + kThr * sizeof(long long);
+           (fs > 1 ? (size_t)nw * vals * 64 * sizeof(long long) : 0) + kThrBytes;
 }
 
 inline bool cb_tr_supported(int A, int ts, int fs, int rt)

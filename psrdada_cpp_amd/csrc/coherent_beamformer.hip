@@ -1559,11 +1559,10 @@ __global__ __launch_bounds__(256 * NTS) void cb_tr_k(CbTrArgs ta)
         const bool live = wave_active && tiles_live;
         // Incoherent-beam partials: in beam group 0, one wave of each time
         // slot per step (whether it holds beams or not) forms the tile's.
-        // The role rotates with the step and the time slot, so that every
-        // SIMD (which holds the waves of one beam-group index) takes one tile
-        // per two-step barrier interval, and a wave issues one partials
-        // store per four steps (stores count against the staging loads'
-        // vmcnt wait).
+        // The role rotates with the step (and, with two time slots, the
+        // slot), so that a wave issues one partials store per four steps
+        // (stores count against the staging loads' vmcnt wait) and the
+        // SIMDs take turns.
         const bool ib_live = IBF && bgroup == 0 && tiles_live;
 
         // Tiles of the pass: NSLOT/SPT groups of tps tiles (tps = 1 unless

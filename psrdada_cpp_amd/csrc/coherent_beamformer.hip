@@ -2615,13 +2615,17 @@ static bool cb_ws_applies(const fbf_cb_desc& d, int mode, long long T)
 // Automatic choice: up to three 32-byte K steps (48 antennas) the
 // wave-private kernel, which measured 20-85 % faster there (8 to 48
 // antennas, tscrunch 4 to 64; one exception, 32 antennas at tscrunch 4, 8 %
-// slower); else the transposed kernel with 64 beams per wave, else with 32
-// beams per wave, else the LDS-staged kernel (which also serves int32 sums,
-// power output and the geometries the transposed kernels do not).
+// slower); else the transposed kernel with 128 beams per wave (fscrunch 1:
+// 3.2 % faster than 64 beams per wave at 64 antennas, 64 channels, 512
+// beams), else with 64, else with 32 beams per wave, else the LDS-staged
+// kernel (which also serves int32 sums, power output and the geometries the
+// transposed kernels do not).
 static CbVariant cb_pick_variant(const fbf_cb_desc& d, int variant, int mode, long long T)
 {
     if (variant <= 0) {
-        if ((2 * d.nantennas + 31) / 32 <= 3 && cb_ws_applies(d, mode, T)) return kCbVariants[kCbVariantWs];
+        const int ksteps = (2 * d.nantennas + 31) / 32;
+        if (ksteps <= 3 && cb_ws_applies(d, mode, T)) return kCbVariants[kCbVariantWs];
+        if (ksteps > 3 && cb_tr_applies(d, 8, mode, T)) return kCbVariants[kCbVariantTr8];
         if (cb_tr_applies(d, 4, mode, T)) return kCbVariants[kCbVariantTr4];
         if (cb_tr_applies(d, 2, mode, T)) return kCbVariants[kCbVariantTr2];
         return kCbVariants[kCbVariantLds];
@@ -2939,12 +2943,10 @@ extern "C" int fbf_cb_beamform_subarray(fbf_cb_plan* plan, const void* ftpa_bloc
 // transposed kernel: the plan's kernel is cb_tr_k with 64 or 128 beams per
 // wave and fscrunch 1, a heap holds a power-of-two number of samples, at least one tile
 // (16 samples), and a heap group's bytes fit a 31-bit offset.
-// The variant fbf_cb_beamform_taftp picks from: the plan's; when automatic
-// and the FTPA pick would be the transposed kernel (more than three K steps),
-// 128 beams per wave where that applies.  That reads each staged tile once
-// per 128 beams, which halves the transposed LDS reads per MFMA against 64
-// beams per wave (134.3 us against 137.3 us at 64 antennas, 64 channels,
-// 8192 samples, 512 beams, tscrunch 16).
+// The variant fbf_cb_beamform_taftp picks from: the plan's; when automatic,
+// 128 beams per wave above three K steps where that applies (the automatic
+// FTPA pick as well).  That reads each staged tile once per 128 beams, which
+// halves the transposed LDS reads per MFMA against 64 beams per wave.
 static int cb_taftp_variant(const fbf_cb_plan* plan, long long T)
 {
     const fbf_cb_desc& d = plan->d;
